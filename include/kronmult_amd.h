@@ -1,0 +1,188 @@
+/*
+ * kronmult_amd.h -- C ABI of the MI355X (gfx950) batched Kronecker mat-vec.
+ *
+ *     Y(:,k) += kron(A_1(k), ..., A_d(k)) * X(:,k),   k = 0 .. batchCount-1,
+ *     d = 1..6.
+ *
+ * Conventions are the reference's (KRONMULT):
+ *   - column-major storage; X and Y are tensors with the last index fastest,
+ *     A_k(i,j) = A_k[i + j*ld] (kroncommon.hpp:72-152, kgemm_nt.hpp:81);
+ *   - every call ACCUMULATES into Y; outputs of different entries may overlap
+ *     (several pY[k] equal), in which case all contributions are summed
+ *     (kgemm_nt.hpp:184-187);
+ *   - all arrays, including the pointer arrays pX/pY/pW/Aarray, live in
+ *     device memory (test_kronmult6_xbatched.cpp:104-133, 268-271);
+ *   - the library allocates nothing on these calls; launches are asynchronous
+ *     on `stream` (a hipStream_t, NULL = default stream), like the
+ *     reference's null-stream launches (kronmult6_batched.cpp:12-14).
+ *
+ * Differences from the reference, all of them supersets:
+ *   - every function returns a status instead of void, after validating its
+ *     scalar arguments on the host;
+ *   - X is never modified (the reference uses X as scratch in batched and
+ *     pbatched, kronm_backward.hpp:128);
+ *   - W / pW are only touched when the requested shape has no on-chip kernel
+ *     (see km_query_kernel); they may be NULL otherwise;
+ *   - options select the rounding order (bitwise reproduction of either
+ *     reference build), a deterministic output-sorted reduction (plans), and
+ *     the stage precision.
+ */
+#ifndef KRONMULT_AMD_H
+#define KRONMULT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum km_status
+{
+  KM_SUCCESS = 0,
+  KM_ERROR_INVALID_VALUE = 1,          /* d, n, lda, batchCount, null pointer */
+  KM_ERROR_INSUFFICIENT_WORKSPACE = 2, /* fallback kernel needs W / pW      */
+  KM_ERROR_NOT_SUPPORTED = 3,          /* shape/option combination not built */
+  KM_ERROR_HIP = 4,                    /* HIP runtime / launch failure       */
+  KM_ERROR_PLAN_MISMATCH = 5           /* plan built for another call shape  */
+} km_status;
+
+/* Order in which the d mode contractions are applied (decides rounding). */
+typedef enum km_variant
+{
+  KM_VARIANT_BACKWARD = 0,  /* modes d..1: Alg. 993 backward,
+                               kronm_backward.hpp:92-139 (default)          */
+  KM_VARIANT_FORWARD = 1,   /* modes 1..d: Alg. 993 forward / the recursive
+                               default build, kronmultv_recur.hpp:90-145     */
+  KM_VARIANT_ALG993 = 2     /* the reference's flop-based choice between the
+                               two, kronm.hpp:83-107 (square => backward)   */
+} km_variant;
+
+/* Arithmetic of the stage contractions. */
+typedef enum km_precision
+{
+  KM_PRECISION_FP64 = 0      /* stages and accumulation in the data type     */
+} km_precision;
+
+/* option flags */
+#define KM_FLAG_EXACT 0x1u        /* separate rounded multiply and add in every
+                                     dot product (the reference's CPU rounding);
+                                     default uses fused multiply-add          */
+#define KM_FLAG_FORCE_GLOBAL 0x2u /* always use the workspace-staged kernel   */
+
+typedef struct km_plan_s *km_plan;
+
+typedef struct km_options
+{
+  unsigned flags;    /* KM_FLAG_* */
+  int variant;       /* km_variant */
+  int precision;     /* km_precision */
+  km_plan plan;      /* optional deterministic reduction plan (km_plan_create) */
+  int reserved[4];
+} km_options;
+
+/* Fill *opt with the defaults (backward order, FMA, FP64, no plan). */
+void km_options_init(km_options *opt);
+
+const char *km_status_string(km_status s);
+
+/* ---------------------------------------------------------------------------
+ * kronmultN_batched  (kronmult6_batched.cpp:4-18, kronmult6_batched.hpp:12-75)
+ *   A: (n, n, d, batchCount) contiguous; X, Y, W: (n^d, batchCount)
+ *   contiguous.  Y(:,k) += kron(A(:,:,1,k), ..., A(:,:,d,k)) X(:,k).
+ * ------------------------------------------------------------------------ */
+km_status km_kronmult_batched_f64(int d, int n, const double *A,
+                                  const double *X, double *Y, double *W,
+                                  int batchCount, void *stream,
+                                  const km_options *opt);
+km_status km_kronmult_batched_f32(int d, int n, const float *A, const float *X,
+                                  float *Y, float *W, int batchCount,
+                                  void *stream, const km_options *opt);
+
+/* ---------------------------------------------------------------------------
+ * kronmultN_pbatched  (kronmult6_pbatched.cpp:4-19, kronmult6_pbatched.hpp:12-67)
+ *   A: (n, n, d, batchCount) contiguous; pX/pY/pW: device arrays of
+ *   batchCount device pointers to vectors of n^d.  pY entries may repeat.
+ * ------------------------------------------------------------------------ */
+km_status km_kronmult_pbatched_f64(int d, int n, const double *A,
+                                   double *const *pX, double *const *pY,
+                                   double *const *pW, int batchCount,
+                                   void *stream, const km_options *opt);
+km_status km_kronmult_pbatched_f32(int d, int n, const float *A,
+                                   float *const *pX, float *const *pY,
+                                   float *const *pW, int batchCount,
+                                   void *stream, const km_options *opt);
+
+/* ---------------------------------------------------------------------------
+ * kronmultN_xbatched  (kronmult6_xbatched.cpp:4-28, kronmult_xbatched.hpp:17-162)
+ *   Aarray: device array of d*batchCount device pointers, A_k of entry b at
+ *   Aarray[k + d*b], each n x n with leading dimension lda >= n (pointers may
+ *   repeat: pooled A).  pX/pY: batchCount pointers (both may repeat).
+ *   pW: workspace pairs as in the reference (pW[2*s], pW[2*s+1] for
+ *   sub-batch slot s < subbatchCount; subbatchCount 0 means
+ *   max(1, batchCount/2), kronmult6_xbatched.cpp:16-20).
+ * ------------------------------------------------------------------------ */
+km_status km_kronmult_xbatched_f64(int d, int n, const double *const *Aarray,
+                                   int lda, double *const *pX,
+                                   double *const *pY, double *const *pW,
+                                   int batchCount, int subbatchCount,
+                                   void *stream, const km_options *opt);
+km_status km_kronmult_xbatched_f32(int d, int n, const float *const *Aarray,
+                                   int lda, float *const *pX, float *const *pY,
+                                   float *const *pW, int batchCount,
+                                   int subbatchCount, void *stream,
+                                   const km_options *opt);
+
+/* ---------------------------------------------------------------------------
+ * kronmultN_vbatched  (kronmult6_vbatched.hpp:10-34, kronmult_vbatched.hpp:17-228)
+ *   Rectangular A_k (m[k] x n[k], leading dimension m[k]), shapes uniform
+ *   over the batch.  Aarray: d*batchCount device pointers.  X of entry b has
+ *   prod(n) elements, Y prod(m).  W: one workspace of Wcapacity_bytes, used
+ *   only by the workspace-staged kernel (two slots of max stage size per
+ *   concurrent entry, kronmult_vbatched.hpp:114-135).  The reference has no
+ *   host wrapper for vbatched; this is that wrapper.
+ * ------------------------------------------------------------------------ */
+km_status km_kronmult_vbatched_f64(int d, const int *m, const int *n,
+                                   const double *const *Aarray,
+                                   double *const *pX, double *const *pY,
+                                   double *W, size_t Wcapacity_bytes,
+                                   int batchCount, void *stream,
+                                   const km_options *opt);
+km_status km_kronmult_vbatched_f32(int d, const int *m, const int *n,
+                                   const float *const *Aarray, float *const *pX,
+                                   float *const *pY, float *W,
+                                   size_t Wcapacity_bytes, int batchCount,
+                                   void *stream, const km_options *opt);
+
+/* ---------------------------------------------------------------------------
+ * Deterministic output-sorted reduction.  A plan records, for one set of
+ * output pointers pY (device array of batchCount pointers), a stable order of
+ * the entries grouped by output and a split of those groups over
+ * workgroups, so that every output vector is owned by exactly one workgroup:
+ * no atomics, bitwise reproducible results run to run.  No counterpart in
+ * the reference (its overlapping outputs use atomics, kgemm_nt.hpp:184-187).
+ * Creating a plan reads pY back to the host, sorts, and allocates two small
+ * device arrays; a call that passes the plan allocates nothing.
+ * elem_bytes is sizeof(double) or sizeof(float).
+ * ------------------------------------------------------------------------ */
+km_status km_plan_create(km_plan *plan, int d, int n, int elem_bytes,
+                         const void *const *pY, int batchCount, void *stream);
+km_status km_plan_destroy(km_plan plan);
+/* number of distinct outputs and of workgroups in the plan */
+km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
+
+/* ---------------------------------------------------------------------------
+ * Introspection: which kernel family a square (d, n) call would use.
+ *   *family: 0 = LDS-resident, 1 = slab-streamed, 2 = workspace-staged.
+ *   *threads: workgroup size.
+ * ------------------------------------------------------------------------ */
+km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
+                          int *family, int *threads);
+
+/* Reference flop model (kron_flops, kroncommon.hpp:154-185) for one entry. */
+double km_kron_flops(int d, const int *m, const int *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KRONMULT_AMD_H */

@@ -1,0 +1,300 @@
+"""kronmult_amd -- MI355X (gfx950) batched Kronecker matrix-vector products.
+
+    Y(:,k) += kron(A_1(k), ..., A_d(k)) X(:,k),   k = 0..batchCount-1,  d = 1..6
+
+Python mirror of the C ABI in ``include/kronmult_amd.h`` (the product is the
+shared library ``libkronmult_amd.so`` built from ``kronmult_amd/csrc``).  The
+functions keep the reference's entry-point names and argument meaning
+(``kronmultN_batched / _pbatched / _xbatched / _vbatched``,
+kronmult6_batched.cpp:4-18, kronmult6_pbatched.cpp:4-19,
+kronmult6_xbatched.cpp:4-28, kronmult6_vbatched.hpp:10-34) with ``N`` given as
+the argument ``d``.  Device memory and streams come from PyTorch; every call is
+asynchronous on the current stream, like the reference's launches.
+
+The library must be built (``__graft_entry__.build()`` or
+``make -C kronmult_amd/csrc``); importing works without it, but every compute
+call raises ``RuntimeError`` -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkronmult_amd.so")
+
+# km_status
+SUCCESS = 0
+ERROR_INVALID_VALUE = 1
+ERROR_INSUFFICIENT_WORKSPACE = 2
+ERROR_NOT_SUPPORTED = 3
+ERROR_HIP = 4
+ERROR_PLAN_MISMATCH = 5
+
+# km_variant
+VARIANT_BACKWARD = 0
+VARIANT_FORWARD = 1
+VARIANT_ALG993 = 2
+
+# km_precision
+PRECISION_FP64 = 0
+
+# flags
+FLAG_EXACT = 0x1
+FLAG_FORCE_GLOBAL = 0x2
+
+# kernel families reported by query_kernel
+FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL = 0, 1, 2
+
+# every symbol the C ABI exports (include/kronmult_amd.h)
+C_ABI_SYMBOLS = (
+    "km_options_init", "km_status_string",
+    "km_kronmult_batched_f64", "km_kronmult_batched_f32",
+    "km_kronmult_pbatched_f64", "km_kronmult_pbatched_f32",
+    "km_kronmult_xbatched_f64", "km_kronmult_xbatched_f32",
+    "km_kronmult_vbatched_f64", "km_kronmult_vbatched_f32",
+    "km_plan_create", "km_plan_destroy", "km_plan_info",
+    "km_query_kernel", "km_kron_flops",
+)
+
+
+class KronmultError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: {status_string(status)} (status {status})")
+
+
+class Options(C.Structure):
+    """``km_options``."""
+    _fields_ = [
+        ("flags", C.c_uint),
+        ("variant", C.c_int),
+        ("precision", C.c_int),
+        ("plan", C.c_void_p),
+        ("reserved", C.c_int * 4),
+    ]
+
+
+_P, _I, _SZ = C.c_void_p, C.c_int, C.c_size_t
+_lib = None
+
+
+def library_loaded() -> bool:
+    return _lib is not None
+
+
+def lib():
+    """Load ``libkronmult_amd.so``; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"kronmult_amd: native library not built ({LIB_PATH}); run "
+                "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        L.km_options_init.argtypes = [C.POINTER(Options)]
+        L.km_options_init.restype = None
+        L.km_status_string.argtypes = [_I]
+        L.km_status_string.restype = C.c_char_p
+        for suf in ("f64", "f32"):
+            f = getattr(L, f"km_kronmult_batched_{suf}")
+            f.argtypes = [_I, _I, _P, _P, _P, _P, _I, _P, C.POINTER(Options)]
+            f.restype = _I
+            f = getattr(L, f"km_kronmult_pbatched_{suf}")
+            f.argtypes = [_I, _I, _P, _P, _P, _P, _I, _P, C.POINTER(Options)]
+            f.restype = _I
+            f = getattr(L, f"km_kronmult_xbatched_{suf}")
+            f.argtypes = [_I, _I, _P, _I, _P, _P, _P, _I, _I, _P, C.POINTER(Options)]
+            f.restype = _I
+            f = getattr(L, f"km_kronmult_vbatched_{suf}")
+            f.argtypes = [_I, _P, _P, _P, _P, _P, _P, _SZ, _I, _P, C.POINTER(Options)]
+            f.restype = _I
+        L.km_plan_create.argtypes = [C.POINTER(C.c_void_p), _I, _I, _I, _P, _I, _P]
+        L.km_plan_create.restype = _I
+        L.km_plan_destroy.argtypes = [_P]
+        L.km_plan_destroy.restype = _I
+        L.km_plan_info.argtypes = [_P, C.POINTER(_I), C.POINTER(_I)]
+        L.km_plan_info.restype = _I
+        L.km_query_kernel.argtypes = [_I, _I, _I, C.POINTER(Options), C.POINTER(_I), C.POINTER(_I)]
+        L.km_query_kernel.restype = _I
+        L.km_kron_flops.argtypes = [_I, _P, _P]
+        L.km_kron_flops.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def status_string(status: int) -> str:
+    try:
+        return lib().km_status_string(int(status)).decode()
+    except RuntimeError:
+        return f"status {status}"
+
+
+def options(exact: bool = False, variant: int = VARIANT_BACKWARD, plan: "Optional[Plan]" = None,
+            force_global: bool = False, precision: int = PRECISION_FP64) -> Options:
+    o = Options()
+    lib().km_options_init(C.byref(o))
+    o.flags = (FLAG_EXACT if exact else 0) | (FLAG_FORCE_GLOBAL if force_global else 0)
+    o.variant = int(variant)
+    o.precision = int(precision)
+    o.plan = plan.handle if plan is not None else None
+    return o
+
+
+def _check(status: int, where: str) -> None:
+    if status != SUCCESS:
+        raise KronmultError(status, where)
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _addr(t) -> Optional[int]:
+    """Device address of a torch tensor, an int address, or None."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _suffix(t) -> str:
+    import torch
+    if t.dtype == torch.float64:
+        return "f64"
+    if t.dtype == torch.float32:
+        return "f32"
+    raise TypeError(f"kronmult_amd: unsupported dtype {t.dtype}")
+
+
+def _opt_ref(opt: Optional[Options]):
+    return C.byref(opt) if opt is not None else None
+
+
+# ---------------------------------------------------------------------------
+# entry points (mirror of the reference's host wrappers)
+# ---------------------------------------------------------------------------
+def kronmult_batched(d: int, n: int, A, X, Y, W=None, batchCount: Optional[int] = None,
+                     stream=None, opt: Optional[Options] = None) -> None:
+    """kronmultN_batched: ``A`` holds the reference's (n,n,d,batch) block
+    (a tensor of shape (batch, d, n, n) indexed [b, k, j, i]); ``X``, ``Y``
+    (and optional ``W``) hold (n^d, batch) column blocks, i.e. tensors of
+    shape (batch, n^d).  Y is accumulated; X is not modified."""
+    if batchCount is None:
+        batchCount = Y.shape[0]
+    f = getattr(lib(), f"km_kronmult_batched_{_suffix(Y)}")
+    _check(f(d, n, _addr(A), _addr(X), _addr(Y), _addr(W), batchCount, _stream(stream), _opt_ref(opt)),
+           f"kronmult{d}_batched")
+
+
+def kronmult_pbatched(d: int, n: int, A, pX, pY, pW, batchCount: int, dtype="f64",
+                      stream=None, opt: Optional[Options] = None) -> None:
+    """kronmultN_pbatched: ``pX``, ``pY``, ``pW`` are device arrays of
+    ``batchCount`` device pointers (int64 tensors, see :func:`pointer_array`);
+    ``pW`` may be None when the shape has an on-chip kernel."""
+    f = getattr(lib(), f"km_kronmult_pbatched_{dtype}")
+    _check(f(d, n, _addr(A), _addr(pX), _addr(pY), _addr(pW), batchCount, _stream(stream), _opt_ref(opt)),
+           f"kronmult{d}_pbatched")
+
+
+def kronmult_xbatched(d: int, n: int, Aarray, lda: int, pX, pY, pW, batchCount: int,
+                      subbatchCount: int = 0, dtype="f64", stream=None,
+                      opt: Optional[Options] = None) -> None:
+    """kronmultN_xbatched: ``Aarray`` is a device array of d*batchCount
+    pointers (A_k of entry b at index k + d*b) to n x n matrices with leading
+    dimension ``lda``."""
+    f = getattr(lib(), f"km_kronmult_xbatched_{dtype}")
+    _check(f(d, n, _addr(Aarray), lda, _addr(pX), _addr(pY), _addr(pW), batchCount, subbatchCount,
+             _stream(stream), _opt_ref(opt)), f"kronmult{d}_xbatched")
+
+
+def kronmult_vbatched(m: Sequence[int], n: Sequence[int], Aarray, pX, pY, W, Wcapacity_bytes: int,
+                      batchCount: int, dtype="f64", stream=None, opt: Optional[Options] = None) -> None:
+    """kronmultN_vbatched with N = len(m): A_k is m[k] x n[k] (leading
+    dimension m[k]); shapes are uniform over the batch."""
+    d = len(m)
+    mm = (C.c_int * d)(*m)
+    nn = (C.c_int * d)(*n)
+    f = getattr(lib(), f"km_kronmult_vbatched_{dtype}")
+    _check(f(d, C.cast(mm, C.c_void_p), C.cast(nn, C.c_void_p), _addr(Aarray), _addr(pX), _addr(pY),
+             _addr(W), int(Wcapacity_bytes), batchCount, _stream(stream), _opt_ref(opt)),
+           f"kronmult{d}_vbatched")
+
+
+# ---------------------------------------------------------------------------
+# plans (deterministic output-sorted reduction)
+# ---------------------------------------------------------------------------
+class Plan:
+    """Deterministic reduction plan for one output pointer array ``pY``."""
+
+    def __init__(self, d: int, n: int, elem_bytes: int, pY, batchCount: int, stream=None):
+        h = C.c_void_p()
+        _check(lib().km_plan_create(C.byref(h), d, n, elem_bytes, _addr(pY), batchCount, _stream(stream)),
+               "km_plan_create")
+        self.handle = h.value
+
+    def info(self):
+        no, ng = C.c_int(), C.c_int()
+        _check(lib().km_plan_info(self.handle, C.byref(no), C.byref(ng)), "km_plan_info")
+        return no.value, ng.value
+
+    def close(self):
+        if self.handle:
+            lib().km_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def query_kernel(d: int, n: int, elem_bytes: int = 8, opt: Optional[Options] = None):
+    """(family, threads) of the kernel a square (d, n) call uses."""
+    fam, nt = C.c_int(), C.c_int()
+    _check(lib().km_query_kernel(d, n, elem_bytes, _opt_ref(opt), C.byref(fam), C.byref(nt)),
+           "km_query_kernel")
+    return fam.value, nt.value
+
+
+def kron_flops(m: Sequence[int], n: Sequence[int]) -> float:
+    """Reference flop model for one entry (kroncommon.hpp:154-185)."""
+    d = len(m)
+    mm = (C.c_int * d)(*m)
+    nn = (C.c_int * d)(*n)
+    return lib().km_kron_flops(d, C.cast(mm, C.c_void_p), C.cast(nn, C.c_void_p))
+
+
+# ---------------------------------------------------------------------------
+# helpers for building device pointer arrays from torch tensors
+# ---------------------------------------------------------------------------
+def pointer_array(base, idx, device=None):
+    """Device int64 tensor of row addresses ``base[idx[b]]`` (base 2-D, rows
+    contiguous).  ``idx`` is any integer sequence or tensor."""
+    import torch
+    idx_t = torch.as_tensor(idx, dtype=torch.int64, device="cpu")
+    addrs = base.data_ptr() + idx_t * (base.stride(0) * base.element_size())
+    return addrs.to(device if device is not None else base.device)
+
+
+def matrix_pointer_table(pools: Sequence, aidx, device=None):
+    """Device int64 tensor of d*batch matrix addresses for xbatched/vbatched:
+    entry b, mode k -> ``pools[k][aidx[b, k]]`` (each pool 3-D, matrices
+    contiguous)."""
+    import torch
+    aidx_t = torch.as_tensor(aidx, dtype=torch.int64, device="cpu")
+    batch, d = aidx_t.shape
+    cols = []
+    for k in range(d):
+        p = pools[k]
+        cols.append(p.data_ptr() + aidx_t[:, k] * (p.stride(0) * p.element_size()))
+    table = torch.stack(cols, dim=1).reshape(-1)
+    return table.to(device if device is not None else pools[0].device)

@@ -1,0 +1,653 @@
+// km_api.cpp -- the C ABI (include/kronmult_amd.h): host-side validation,
+// kernel selection, launch geometry and deterministic-reduction plans.
+//
+// Entry points replace the reference's host wrappers:
+//   km_kronmult_batched_*   kronmult6_batched.cpp:4-18   (and N = 1..5)
+//   km_kronmult_pbatched_*  kronmult6_pbatched.cpp:4-19
+//   km_kronmult_xbatched_*  kronmult6_xbatched.cpp:4-28
+//   km_kronmult_vbatched_*  kronmult6_vbatched.hpp:10-34 (template only there)
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <utility>
+#include <vector>
+
+#include "../../include/kronmult_amd.h"
+#include "km_internal.h"
+
+using km::EntryArgs;
+using km::KernelInfo;
+
+struct km_plan_s
+{
+  int d = 0, n = 0, elem_bytes = 0, batch = 0;
+  int device = 0;
+  int ngroups = 0;
+  int noutputs = 0;
+  int *order = nullptr;  // device, batch entries
+  int *ranges = nullptr; // device, ngroups + 1 entries
+};
+
+namespace
+{
+
+std::mutex g_mu;
+
+int device_cus()
+{
+  static std::map<int, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return 0;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = cache.find(dev);
+  if (it != cache.end())
+    return it->second;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) !=
+      hipSuccess)
+    v = 0;
+  cache[dev] = v;
+  return v;
+}
+
+int occupancy(const void *fn, int nt)
+{
+  static std::map<std::pair<int, const void *>, int> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = cache.find({dev, fn});
+    if (it != cache.end())
+      return it->second;
+  }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nt, 0) !=
+          hipSuccess ||
+      nb < 1)
+    nb = 1;
+  std::lock_guard<std::mutex> lk(g_mu);
+  cache[{dev, fn}] = nb;
+  return nb;
+}
+
+long long ipow(int b, int e)
+{
+  long long p = 1;
+  for (int i = 0; i < e; i++)
+    p *= b;
+  return p;
+}
+
+// kronm.hpp:37-89: forward only when strictly cheaper than backward
+bool alg993_forward(int d, const int *m, const int *n)
+{
+  double xs = 1;
+  for (int i = 0; i < d; i++)
+    xs *= n[i];
+  double fwd = 0, x = xs;
+  for (int i = 0; i < d; i++)
+  {
+    double nn = x / n[i];
+    fwd += 2.0 * m[i] * nn * n[i];
+    x = m[i] * nn;
+  }
+  double bwd = 0;
+  x = xs;
+  for (int i = d - 1; i >= 0; i--)
+  {
+    double mm = x / n[i];
+    bwd += 2.0 * mm * m[i] * n[i];
+    x = mm * m[i];
+  }
+  return fwd < bwd;
+}
+
+struct Resolved
+{
+  bool exact = false;
+  bool fwd = false;
+  bool force_global = false;
+  km_plan plan = nullptr;
+};
+
+km_status resolve(const km_options *opt, int d, const int *m, const int *n,
+                  Resolved *r)
+{
+  km_options o;
+  km_options_init(&o);
+  if (opt)
+    o = *opt;
+  if (o.precision != KM_PRECISION_FP64)
+    return KM_ERROR_NOT_SUPPORTED;
+  r->exact = (o.flags & KM_FLAG_EXACT) != 0;
+  r->force_global = (o.flags & KM_FLAG_FORCE_GLOBAL) != 0;
+  switch (o.variant)
+  {
+  case KM_VARIANT_BACKWARD: r->fwd = false; break;
+  case KM_VARIANT_FORWARD: r->fwd = true; break;
+  case KM_VARIANT_ALG993: r->fwd = alg993_forward(d, m, n); break;
+  default: return KM_ERROR_INVALID_VALUE;
+  }
+  r->plan = o.plan;
+  return KM_SUCCESS;
+}
+
+// Square on-chip kernel for (d, n) honoring the rounding order, if any.
+template<typename T>
+bool pick_onchip(int d, int n, const Resolved &r, KernelInfo *k)
+{
+  if (r.force_global)
+    return false;
+  // without EXACT the contraction order only changes rounding, so the
+  // forward request may use any kernel
+  bool fwd = r.fwd && r.exact;
+  if (km::lds_kernel<T>(d, n, r.exact, fwd, k))
+    return true;
+  if (!fwd && km::slab_kernel<T>(d, n, r.exact, k))
+    return true;
+  return false;
+}
+
+// Launch description common to all entry points.
+template<typename T>
+struct Call
+{
+  EntryArgs<T> a{};
+  km::Shape sh{};
+  bool square = true;
+  long long xsize = 0; // elements of X per entry
+  // workspace slots available to the staged kernel (0 = none)
+  long long slots = 0;
+  int elem_bytes = sizeof(T);
+};
+
+template<typename T>
+km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
+{
+  const int batch = c.a.batch;
+  if (batch == 0)
+    return KM_SUCCESS;
+  KernelInfo k;
+  bool onchip = c.square && pick_onchip<T>(c.sh.d, c.sh.n[0], r, &k);
+  if (!onchip)
+  {
+    for (int i = 0; i < c.sh.d; i++)
+      if (c.sh.n[i] > KM_GLOBAL_MAX_N)
+        return KM_ERROR_NOT_SUPPORTED;
+    if (c.slots < 1)
+      return KM_ERROR_INSUFFICIENT_WORKSPACE;
+    k = km::global_kernel<T>(r.exact);
+  }
+  int cus = device_cus();
+  if (cus <= 0)
+    return KM_ERROR_HIP;
+  int grid = 0;
+  if (r.plan)
+  {
+    const km_plan p = r.plan;
+    if (p->batch != batch || p->elem_bytes != (int)sizeof(T) ||
+        (c.square && (p->d != c.sh.d || p->n != c.sh.n[0])))
+      return KM_ERROR_PLAN_MISMATCH;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev != p->device)
+      return KM_ERROR_PLAN_MISMATCH;
+    grid = p->ngroups;
+    if (!onchip && grid > c.slots)
+      return KM_ERROR_INSUFFICIENT_WORKSPACE;
+    c.a.ranges = p->ranges;
+    c.a.order = p->order;
+    c.a.owned = 1;
+  }
+  else
+  {
+    long long g = (long long)cus * occupancy(k.fn, k.nt);
+    if (!onchip)
+      g = std::min(g, c.slots);
+    g = std::max(1LL, std::min<long long>(g, batch));
+    int chunk = (int)((batch + g - 1) / g);
+    grid = (batch + chunk - 1) / chunk;
+    c.a.chunk = chunk;
+    c.a.ranges = nullptr;
+    c.a.order = nullptr;
+    c.a.owned = 0;
+  }
+  hipError_t e;
+  if (onchip)
+  {
+    void *args[] = {&c.a};
+    e = hipLaunchKernel(k.fn, dim3(grid), dim3(k.nt), args, 0, stream);
+  }
+  else
+  {
+    int fwd = r.fwd ? 1 : 0;
+    void *args[] = {&c.a, &c.sh, &fwd};
+    e = hipLaunchKernel(k.fn, dim3(grid), dim3(k.nt), args, 0, stream);
+  }
+  if (e != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return KM_ERROR_HIP;
+  }
+  return KM_SUCCESS;
+}
+
+km_status check_dn(int d, int n, int batch)
+{
+  if (d < 1 || d > 6 || n < 1 || batch < 0)
+    return KM_ERROR_INVALID_VALUE;
+  if (ipow(n, d) > (1LL << 31) - 1)
+    return KM_ERROR_INVALID_VALUE;
+  return KM_SUCCESS;
+}
+
+void square_shape(km::Shape &sh, int d, int n)
+{
+  sh.d = d;
+  for (int k = 0; k < 6; k++)
+  {
+    sh.m[k] = (k < d) ? n : 1;
+    sh.n[k] = (k < d) ? n : 1;
+  }
+}
+
+template<typename T>
+km_status batched(int d, int n, const T *A, const T *X, T *Y, T *W, int batch,
+                  void *stream, const km_options *opt)
+{
+  km_status s = check_dn(d, n, batch);
+  if (s != KM_SUCCESS)
+    return s;
+  if (batch > 0 && (!A || !X || !Y))
+    return KM_ERROR_INVALID_VALUE;
+  Call<T> c;
+  square_shape(c.sh, d, n);
+  Resolved r;
+  if ((s = resolve(opt, d, c.sh.m, c.sh.n, &r)) != KM_SUCCESS)
+    return s;
+  const long long xs = ipow(n, d);
+  c.xsize = xs;
+  c.a.A = A;
+  c.a.a_stride = (long long)n * n * d;
+  c.a.a_kstride = (long long)n * n;
+  c.a.lda = n;
+  c.a.X = X;
+  c.a.x_stride = xs;
+  c.a.Y = Y;
+  c.a.y_stride = xs;
+  c.a.W = W;
+  c.a.w_stride = xs;
+  c.a.w_half = 0;
+  c.a.ws_mul = 1;
+  c.a.batch = batch;
+  c.slots = W ? batch : 0;
+  return launch(c, r, (hipStream_t)stream);
+}
+
+template<typename T>
+km_status pbatched(int d, int n, const T *A, T *const *pX, T *const *pY,
+                   T *const *pW, int batch, void *stream,
+                   const km_options *opt)
+{
+  km_status s = check_dn(d, n, batch);
+  if (s != KM_SUCCESS)
+    return s;
+  if (batch > 0 && (!A || !pX || !pY))
+    return KM_ERROR_INVALID_VALUE;
+  Call<T> c;
+  square_shape(c.sh, d, n);
+  Resolved r;
+  if ((s = resolve(opt, d, c.sh.m, c.sh.n, &r)) != KM_SUCCESS)
+    return s;
+  c.xsize = ipow(n, d);
+  c.a.A = A;
+  c.a.a_stride = (long long)n * n * d;
+  c.a.a_kstride = (long long)n * n;
+  c.a.lda = n;
+  c.a.pX = pX;
+  c.a.pY = pY;
+  c.a.pW = pW;
+  c.a.ws_mul = 1;
+  c.a.batch = batch;
+  c.slots = pW ? batch : 0;
+  return launch(c, r, (hipStream_t)stream);
+}
+
+template<typename T>
+km_status xbatched(int d, int n, const T *const *Aarray, int lda, T *const *pX,
+                   T *const *pY, T *const *pW, int batch, int sub,
+                   void *stream, const km_options *opt)
+{
+  km_status s = check_dn(d, n, batch);
+  if (s != KM_SUCCESS)
+    return s;
+  if (lda < n || sub < 0)
+    return KM_ERROR_INVALID_VALUE;
+  if (batch > 0 && (!Aarray || !pX || !pY))
+    return KM_ERROR_INVALID_VALUE;
+  Call<T> c;
+  square_shape(c.sh, d, n);
+  Resolved r;
+  if ((s = resolve(opt, d, c.sh.m, c.sh.n, &r)) != KM_SUCCESS)
+    return s;
+  if (sub == 0)
+    sub = std::max(1, batch / 2); // kronmult6_xbatched.cpp:16-20
+  c.xsize = ipow(n, d);
+  c.a.Aptr = Aarray;
+  c.a.lda = lda;
+  c.a.pX = pX;
+  c.a.pY = pY;
+  c.a.pW = pW;
+  c.a.ws_mul = 2; // slot s uses the pair pW[2s], pW[2s+1]
+  c.a.batch = batch;
+  c.slots = pW ? sub : 0;
+  return launch(c, r, (hipStream_t)stream);
+}
+
+template<typename T>
+km_status vbatched(int d, const int *m, const int *n, const T *const *Aarray,
+                   T *const *pX, T *const *pY, T *W, size_t Wcap, int batch,
+                   void *stream, const km_options *opt)
+{
+  if (d < 1 || d > 6 || batch < 0 || !m || !n)
+    return KM_ERROR_INVALID_VALUE;
+  for (int k = 0; k < d; k++)
+    if (m[k] < 1 || n[k] < 1)
+      return KM_ERROR_INVALID_VALUE;
+  if (batch > 0 && (!Aarray || !pX || !pY))
+    return KM_ERROR_INVALID_VALUE;
+  Call<T> c;
+  c.sh.d = d;
+  bool square = true;
+  for (int k = 0; k < 6; k++)
+  {
+    c.sh.m[k] = (k < d) ? m[k] : 1;
+    c.sh.n[k] = (k < d) ? n[k] : 1;
+    if (k < d)
+      square = square && (m[k] == n[0] && n[k] == n[0]);
+  }
+  c.square = square;
+  Resolved r;
+  km_status s;
+  if ((s = resolve(opt, d, c.sh.m, c.sh.n, &r)) != KM_SUCCESS)
+    return s;
+  // Largest intermediate of the stage order in use (for the forward order
+  // this is the reference's sizeW, kronmult_vbatched.hpp:114-130; the
+  // backward order contracts the last modes first).
+  long long sx = 1, sw = 0;
+  for (int k = 0; k < d; k++)
+    sx *= n[k];
+  for (int L = 1; L < d; L++)
+  {
+    long long v = 1;
+    for (int k = 0; k < d; k++)
+    {
+      bool done = r.fwd ? (k < L) : (k >= d - L);
+      v *= done ? m[k] : n[k];
+    }
+    sw = std::max(sw, v);
+  }
+  long long sy = 1;
+  for (int k = 0; k < d; k++)
+    sy *= m[k];
+  if (sx > (1LL << 31) - 1 || sy > (1LL << 31) - 1 || sw > (1LL << 31) - 1)
+    return KM_ERROR_INVALID_VALUE;
+  const long long sxw = std::max(std::max(sx, sw), sy);
+  c.xsize = sx;
+  c.a.Aptr = Aarray;
+  c.a.lda = square ? m[0] : 0; // 0: A_k has leading dimension m_k
+  c.a.pX = pX;
+  c.a.pY = pY;
+  c.a.W = W;
+  c.a.w_stride = 2 * sxw;
+  c.a.w_half = sxw;
+  c.a.batch = batch;
+  c.slots = W ? (long long)(Wcap / (2 * sxw * sizeof(T))) : 0;
+  return launch(c, r, (hipStream_t)stream);
+}
+
+} // namespace
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+extern "C"
+{
+
+  void km_options_init(km_options *opt)
+  {
+    if (!opt)
+      return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->flags = 0;
+    opt->variant = KM_VARIANT_BACKWARD;
+    opt->precision = KM_PRECISION_FP64;
+    opt->plan = nullptr;
+  }
+
+  const char *km_status_string(km_status s)
+  {
+    switch (s)
+    {
+    case KM_SUCCESS: return "success";
+    case KM_ERROR_INVALID_VALUE: return "invalid value";
+    case KM_ERROR_INSUFFICIENT_WORKSPACE: return "insufficient workspace";
+    case KM_ERROR_NOT_SUPPORTED: return "not supported";
+    case KM_ERROR_HIP: return "HIP runtime error";
+    case KM_ERROR_PLAN_MISMATCH: return "plan does not match the call";
+    }
+    return "unknown status";
+  }
+
+  km_status km_kronmult_batched_f64(int d, int n, const double *A,
+                                    const double *X, double *Y, double *W,
+                                    int batchCount, void *stream,
+                                    const km_options *opt)
+  {
+    return batched<double>(d, n, A, X, Y, W, batchCount, stream, opt);
+  }
+  km_status km_kronmult_batched_f32(int d, int n, const float *A,
+                                    const float *X, float *Y, float *W,
+                                    int batchCount, void *stream,
+                                    const km_options *opt)
+  {
+    return batched<float>(d, n, A, X, Y, W, batchCount, stream, opt);
+  }
+
+  km_status km_kronmult_pbatched_f64(int d, int n, const double *A,
+                                     double *const *pX, double *const *pY,
+                                     double *const *pW, int batchCount,
+                                     void *stream, const km_options *opt)
+  {
+    return pbatched<double>(d, n, A, pX, pY, pW, batchCount, stream, opt);
+  }
+  km_status km_kronmult_pbatched_f32(int d, int n, const float *A,
+                                     float *const *pX, float *const *pY,
+                                     float *const *pW, int batchCount,
+                                     void *stream, const km_options *opt)
+  {
+    return pbatched<float>(d, n, A, pX, pY, pW, batchCount, stream, opt);
+  }
+
+  km_status km_kronmult_xbatched_f64(int d, int n,
+                                     const double *const *Aarray, int lda,
+                                     double *const *pX, double *const *pY,
+                                     double *const *pW, int batchCount,
+                                     int subbatchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return xbatched<double>(d, n, Aarray, lda, pX, pY, pW, batchCount,
+                            subbatchCount, stream, opt);
+  }
+  km_status km_kronmult_xbatched_f32(int d, int n, const float *const *Aarray,
+                                     int lda, float *const *pX,
+                                     float *const *pY, float *const *pW,
+                                     int batchCount, int subbatchCount,
+                                     void *stream, const km_options *opt)
+  {
+    return xbatched<float>(d, n, Aarray, lda, pX, pY, pW, batchCount,
+                           subbatchCount, stream, opt);
+  }
+
+  km_status km_kronmult_vbatched_f64(int d, const int *m, const int *n,
+                                     const double *const *Aarray,
+                                     double *const *pX, double *const *pY,
+                                     double *W, size_t Wcapacity_bytes,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return vbatched<double>(d, m, n, Aarray, pX, pY, W, Wcapacity_bytes,
+                            batchCount, stream, opt);
+  }
+  km_status km_kronmult_vbatched_f32(int d, const int *m, const int *n,
+                                     const float *const *Aarray,
+                                     float *const *pX, float *const *pY,
+                                     float *W, size_t Wcapacity_bytes,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return vbatched<float>(d, m, n, Aarray, pX, pY, W, Wcapacity_bytes,
+                           batchCount, stream, opt);
+  }
+
+  km_status km_plan_create(km_plan *plan, int d, int n, int elem_bytes,
+                           const void *const *pY, int batchCount,
+                           void *stream)
+  {
+    if (!plan || (elem_bytes != 4 && elem_bytes != 8) || batchCount < 1 ||
+        !pY)
+      return KM_ERROR_INVALID_VALUE;
+    *plan = nullptr;
+    km_status s = check_dn(d, n, batchCount);
+    if (s != KM_SUCCESS)
+      return s;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<uintptr_t> ys(batchCount);
+    if (hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return KM_ERROR_HIP;
+    std::vector<int> order(batchCount);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return ys[x] < ys[y]; });
+    // workgroup count the default kernel would use
+    Resolved r;
+    KernelInfo k;
+    bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
+                                    : pick_onchip<float>(d, n, r, &k);
+    if (!onchip)
+      k = (elem_bytes == 8) ? km::global_kernel<double>(false)
+                            : km::global_kernel<float>(false);
+    long long G = (long long)device_cus() * occupancy(k.fn, k.nt);
+    G = std::max(1LL, std::min<long long>(G, batchCount));
+    const long long target = (batchCount + G - 1) / G;
+    std::vector<int> ranges;
+    ranges.push_back(0);
+    int noutputs = 0;
+    long long in_group = 0;
+    for (int i = 0; i < batchCount;)
+    {
+      int j = i + 1;
+      while (j < batchCount && ys[order[j]] == ys[order[i]])
+        j++;
+      noutputs++;
+      if (in_group > 0 && in_group + (j - i) > target)
+      {
+        ranges.push_back(i);
+        in_group = 0;
+      }
+      in_group += j - i;
+      i = j;
+    }
+    ranges.push_back(batchCount);
+    km_plan p = new km_plan_s;
+    p->d = d;
+    p->n = n;
+    p->elem_bytes = elem_bytes;
+    p->batch = batchCount;
+    (void)hipGetDevice(&p->device);
+    p->ngroups = (int)ranges.size() - 1;
+    p->noutputs = noutputs;
+    if (hipMalloc(&p->order, sizeof(int) * batchCount) != hipSuccess ||
+        hipMalloc(&p->ranges, sizeof(int) * ranges.size()) != hipSuccess ||
+        hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+    {
+      km_plan_destroy(p);
+      return KM_ERROR_HIP;
+    }
+    *plan = p;
+    return KM_SUCCESS;
+  }
+
+  km_status km_plan_destroy(km_plan plan)
+  {
+    if (!plan)
+      return KM_SUCCESS;
+    if (plan->order)
+      (void)hipFree(plan->order);
+    if (plan->ranges)
+      (void)hipFree(plan->ranges);
+    delete plan;
+    return KM_SUCCESS;
+  }
+
+  km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups)
+  {
+    if (!plan)
+      return KM_ERROR_INVALID_VALUE;
+    if (num_outputs)
+      *num_outputs = plan->noutputs;
+    if (num_groups)
+      *num_groups = plan->ngroups;
+    return KM_SUCCESS;
+  }
+
+  km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
+                            int *family, int *threads)
+  {
+    km_status s = check_dn(d, n, 1);
+    if (s != KM_SUCCESS || (elem_bytes != 4 && elem_bytes != 8))
+      return KM_ERROR_INVALID_VALUE;
+    int m[6], nn[6];
+    for (int k = 0; k < 6; k++)
+      m[k] = nn[k] = n;
+    Resolved r;
+    if ((s = resolve(opt, d, m, nn, &r)) != KM_SUCCESS)
+      return s;
+    KernelInfo k;
+    bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
+                                    : pick_onchip<float>(d, n, r, &k);
+    if (!onchip)
+      k = (elem_bytes == 8) ? km::global_kernel<double>(r.exact)
+                            : km::global_kernel<float>(r.exact);
+    if (family)
+      *family = k.family;
+    if (threads)
+      *threads = k.nt;
+    return KM_SUCCESS;
+  }
+
+  double km_kron_flops(int d, const int *m, const int *n)
+  {
+    // kroncommon.hpp:154-185: stage 1 applies A_1 to n_2..n_d vectors of
+    // length n_1, then m_1 copies of the (d-1)-dimensional problem follow
+    if (d < 1 || !m || !n)
+      return 0;
+    if (d == 1)
+      return 2.0 * m[0] * n[0];
+    double rest = 1;
+    for (int k = 1; k < d; k++)
+      rest *= n[k];
+    return 2.0 * rest * n[0] * m[0] + km_kron_flops(d - 1, m + 1, n + 1) * m[0];
+  }
+
+} // extern "C"

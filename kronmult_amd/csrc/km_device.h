@@ -1,0 +1,600 @@
+// km_device.h -- gfx950 device code for the batched Kronecker mat-vec
+//     Y(:,b) += kron(A_1(b), ..., A_d(b)) * X(:,b),   b = 0..batch-1.
+//
+// Index convention (the reference's, test_kronmult6_batched.cpp:352-385):
+// X and Y are column-major tensors with j_d fastest and j_1 slowest; A_k is
+// column-major, A_k(i,j) = Ak[i + j*lda] (kgemm_nt.hpp:81).  Mode k (0-based)
+// of a tensor of n^d elements therefore has stride n^(d-1-k).
+//
+// Three kernel families, all without any global workspace of their own:
+//   kron_lds   whole n^d tensor of the entry in LDS; every mode contraction is
+//              a pass of per-thread fibers (n loads, n*n FMAs, n stores);
+//   kron_slab  n^d too large for LDS but n^(d-1) is not: the input is streamed
+//              in slabs X(j1,:) (direct-to-LDS loads, double buffered), modes
+//              2..d are contracted inside the slab and mode 1 is accumulated
+//              straight into per-thread registers;
+//   kron_global  any shape; stages through the caller's workspace in global
+//              memory (the reference's own storage scheme) -- the fallback.
+//
+// Output handling shared by all families: a workgroup walks a contiguous range
+// of entries (in plan order when a plan is given).  Consecutive entries that
+// target the same Y are summed on chip; a run is flushed with one 64-bit
+// atomic add per element (or a plain read-modify-write when the plan
+// guarantees that the workgroup owns every entry of that output).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace km
+{
+
+// largest mode extent n_k accepted by the workspace-staged kernel
+#define KM_GLOBAL_MAX_N 32
+
+// ---------------------------------------------------------------------------
+// arguments
+// ---------------------------------------------------------------------------
+template<typename T>
+struct EntryArgs
+{
+  // A_k of entry b: Aptr ? Aptr[k + d*b] : A + b*a_stride + k*a_kstride
+  const T *A;
+  const T *const *Aptr;
+  long long a_stride;
+  long long a_kstride;
+  int lda;
+  // X of entry b: pX ? pX[b] : X + b*x_stride
+  const T *X;
+  const T *const *pX;
+  long long x_stride;
+  // Y of entry b: pY ? pY[b] : Y + b*y_stride
+  T *Y;
+  T *const *pY;
+  long long y_stride;
+  // workspace (kron_global only): slot s buffers are W0(s), W1(s)
+  T *const *pW;       // pW[ws_mul*s + 0], pW[ws_mul*s + 1]
+  T *W;               // W + s*w_stride, W + s*w_stride + w_half
+  long long w_stride;
+  long long w_half;
+  int ws_mul;         // 1: pW[s] (one buffer per slot), 2: pW pairs
+  int batch;
+  int chunk;          // entries per workgroup when ranges == nullptr
+  const int *ranges;  // workgroup w walks order positions [ranges[w], ranges[w+1])
+  const int *order;   // order position -> entry
+  int owned;          // 1: workgroup owns its outputs (no atomics)
+};
+
+struct Shape
+{
+  int d;
+  int m[6];
+  int n[6];
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+
+// Address-space views.  Pointer tables, plan arrays and A are read-only for
+// the whole launch, so they are read through the constant address space
+// (scalar loads into SGPRs when the address is wave-uniform); X, Y and the
+// workspace are read and written through the global address space.
+template<typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template<typename T>
+using gptr = __attribute__((address_space(1))) T *;
+
+template<typename T>
+__device__ __forceinline__ T cld(const T *p)
+{
+  return *(cptr<T>)(p);
+}
+
+template<typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+  return *(gptr<const T>)(p);
+}
+
+template<typename T>
+__device__ __forceinline__ void gst(T *p, T v)
+{
+  *(gptr<T>)(p) = v;
+}
+
+template<typename T>
+__device__ __forceinline__ void gatomic_add(T *p, T v)
+{
+  __hip_atomic_fetch_add((gptr<T>)(p), v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template<typename T>
+__device__ __forceinline__ void wg_range(const EntryArgs<T> &a, int &p0, int &p1)
+{
+  if (a.ranges)
+  {
+    p0 = cld(a.ranges + blockIdx.x);
+    p1 = cld(a.ranges + blockIdx.x + 1);
+  }
+  else
+  {
+    p0 = blockIdx.x * a.chunk;
+    p1 = min(a.batch, p0 + a.chunk);
+  }
+}
+
+template<typename T>
+__device__ __forceinline__ int entry_at(const EntryArgs<T> &a, int p)
+{
+  return a.order ? cld(a.order + p) : p;
+}
+
+template<typename T>
+__device__ __forceinline__ const T *x_of(const EntryArgs<T> &a, int b)
+{
+  return a.pX ? cld(a.pX + b) : a.X + (long long)b * a.x_stride;
+}
+
+template<typename T>
+__device__ __forceinline__ T *y_of(const EntryArgs<T> &a, int b)
+{
+  return a.pY ? cld(a.pY + b) : a.Y + (long long)b * a.y_stride;
+}
+
+template<typename T>
+__device__ __forceinline__ const T *a_of(const EntryArgs<T> &a, int b, int k,
+                                         int d)
+{
+  return a.Aptr ? cld(a.Aptr + (k + d * b))
+                : a.A + (long long)b * a.a_stride + k * a.a_kstride;
+}
+
+// acc += a*x.  EXACT: one rounded multiply then one rounded add, the
+// reference's CPU rounding (kgemm_nt.hpp:125-134; this library is compiled
+// with -ffp-contract=off so nothing fuses them).  Otherwise a fused FMA.
+template<bool EXACT, typename T>
+__device__ __forceinline__ T madd(T acc, T a, T x)
+{
+  if constexpr (EXACT)
+  {
+    T p = a * x;
+    return acc + p;
+  }
+  else
+  {
+    return __builtin_fma(a, x, acc);
+  }
+}
+
+// Float entry points accumulate each dot product in double and round the
+// stored result to float, as the reference's float templates do
+// (kgemm_nt.hpp:213-221).
+template<typename T>
+struct Acc
+{
+  using type = T;
+};
+template<>
+struct Acc<float>
+{
+  using type = double;
+};
+
+// Load A_k (n x n, leading dimension lda) into registers.  The pointer is
+// wave-uniform, so these become scalar loads and the values live in SGPRs.
+template<int N, typename T>
+__device__ __forceinline__ void load_a(typename Acc<T>::type (&am)[N][N],
+                                       const T *Ak, int lda)
+{
+#pragma unroll
+  for (int j = 0; j < N; j++)
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      am[i][j] = static_cast<typename Acc<T>::type>(cld(Ak + (i + j * lda)));
+}
+
+// out_i = sum_j am[i][j] * x_j over j ascending (started from zero as in the
+// reference), written back over the fiber's own n slots.
+template<int N, bool EXACT, typename T>
+__device__ __forceinline__ void fiber_apply(T *p, int s,
+                                            const typename Acc<T>::type (&am)[N][N])
+{
+  using C = typename Acc<T>::type;
+  C x[N];
+#pragma unroll
+  for (int j = 0; j < N; j++)
+    x[j] = static_cast<C>(p[j * s]);
+#pragma unroll
+  for (int i = 0; i < N; i++)
+  {
+    C acc = 0;
+    if constexpr (EXACT)
+    {
+#pragma unroll
+      for (int j = 0; j < N; j++)
+        acc = madd<true>(acc, am[i][j], x[j]);
+    }
+    else
+    {
+      acc = am[i][0] * x[0];
+#pragma unroll
+      for (int j = 1; j < N; j++)
+        acc = madd<false>(acc, am[i][j], x[j]);
+    }
+    p[i * s] = static_cast<T>(acc);
+  }
+}
+
+template<typename T>
+__device__ __forceinline__ void flush_add(T *dst, T v, int owned)
+{
+  if (owned)
+    gst(dst, gld(dst) + v);
+  else
+    gatomic_add(dst, v);
+}
+
+// ---------------------------------------------------------------------------
+// kron_lds: whole tensor in LDS, one workgroup per entry at a time.
+//   FWD = false: modes contracted d..1 (Alg. 993 backward order,
+//                kronm_backward.hpp:92-139)
+//   FWD = true : modes contracted 1..d (the recursive / forward order,
+//                kronmultv_recur.hpp:90-145, kronm_forward.hpp:110-164)
+// ---------------------------------------------------------------------------
+template<typename T, int D, int N, int NT, bool EXACT, bool FWD, int K>
+__device__ __forceinline__ void lds_stage(T *buf, const EntryArgs<T> &a, int b,
+                                          int tid)
+{
+  using C = typename Acc<T>::type;
+  constexpr int S = cpow(N, D);
+  constexpr int F = S / N;
+  constexpr int MODE = FWD ? K : D - 1 - K;
+  constexpr int STRIDE = cpow(N, D - 1 - MODE);
+  C am[N][N];
+  load_a<N>(am, a_of(a, b, MODE, D), a.lda);
+#pragma unroll 2
+  for (int f = tid; f < F; f += NT)
+  {
+    int hi = f / STRIDE;
+    int lo = f - hi * STRIDE;
+    fiber_apply<N, EXACT, T>(buf + hi * STRIDE * N + lo, STRIDE, am);
+  }
+  __syncthreads();
+  if constexpr (K + 1 < D)
+    lds_stage<T, D, N, NT, EXACT, FWD, K + 1>(buf, a, b, tid);
+}
+
+template<typename T, int D, int N, int NT, bool EXACT, bool FWD>
+__global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
+{
+  constexpr int S = cpow(N, D);
+  constexpr int R = (S + NT - 1) / NT;
+  __shared__ T buf[S];
+  const int tid = threadIdx.x;
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+
+  T acc[R];
+  T xr[R];
+  T *target = nullptr;
+
+  {
+    const T *xp = x_of(a, entry_at(a, p0));
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      int q = tid + NT * r;
+      if (S % NT == 0 || q < S)
+        xr[r] = gld(xp + q);
+    }
+  }
+
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    T *yb = y_of(a, b);
+    __syncthreads(); // buf free: previous entry fully consumed
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      int q = tid + NT * r;
+      if (S % NT == 0 || q < S)
+        buf[q] = xr[r];
+    }
+    if (p + 1 < p1)
+    {
+      const T *xp = x_of(a, entry_at(a, p + 1));
+#pragma unroll
+      for (int r = 0; r < R; r++)
+      {
+        int q = tid + NT * r;
+        if (S % NT == 0 || q < S)
+          xr[r] = gld(xp + q);
+      }
+    }
+    if (yb != target)
+    {
+      if (target)
+      {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+        {
+          int q = tid + NT * r;
+          if (S % NT == 0 || q < S)
+            flush_add(target + q, acc[r], a.owned);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        acc[r] = 0;
+      target = yb;
+    }
+    __syncthreads();
+    lds_stage<T, D, N, NT, EXACT, FWD, 0>(buf, a, b, tid);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      int q = tid + NT * r;
+      if (S % NT == 0 || q < S)
+        acc[r] = acc[r] + buf[q];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++)
+  {
+    int q = tid + NT * r;
+    if (S % NT == 0 || q < S)
+      flush_add(target + q, acc[r], a.owned);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kron_slab: X streamed in slabs X(j1, :) of n^(d-1) elements.
+//   Y(i1, m) = sum_j1 A_1(i1, j1) * [kron(A_2..A_d) X(j1, :)](m)
+// Inside a slab the modes d..2 are contracted in LDS (backward order); the
+// mode-1 sum runs over j1 ascending straight into registers, so the order of
+// every dot product is the Alg. 993 backward order.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void *lds_vptr;
+typedef __attribute__((address_space(1))) void *gbl_vptr;
+
+// Copy SL elements (SL*sizeof(T) a multiple of 16, src 16-byte aligned) from
+// global memory straight into LDS, 16 bytes per lane.  Each wave writes one
+// contiguous 1 KiB block per instruction.
+template<typename T, int SL, int NT>
+__device__ __forceinline__ void slab_async_load(T *dst, const T *src, int tid)
+{
+  constexpr int CH = SL * (int)sizeof(T) / 16; // 16-byte chunks
+  constexpr int IT = (CH + NT - 1) / NT;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+#pragma unroll
+  for (int it = 0; it < IT; it++)
+  {
+    int c0 = it * NT + wave * 64; // first chunk of this wave
+    if (c0 < CH)
+    {
+      int c = c0 + lane;
+      if (CH % 64 == 0 || c < CH)
+      {
+        const char *g = reinterpret_cast<const char *>(src) + (size_t)c * 16;
+        char *l = reinterpret_cast<char *>(dst) + (size_t)c0 * 16;
+        __builtin_amdgcn_global_load_lds((gbl_vptr)(g), (lds_vptr)(l), 16, 0, 0);
+      }
+    }
+  }
+}
+
+template<typename T, int D, int N, int NT, bool EXACT, int K>
+__device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
+                                           int tid)
+{
+  // K runs 0..D-2 over in-slab modes D-1 .. 1 (global mode numbering)
+  using C = typename Acc<T>::type;
+  constexpr int SL = cpow(N, D - 1);
+  constexpr int F = SL / N;
+  constexpr int MODE = D - 1 - K;
+  constexpr int STRIDE = cpow(N, D - 1 - MODE);
+  C am[N][N];
+  load_a<N>(am, a_of(a, b, MODE, D), a.lda);
+#pragma unroll 1
+  for (int f = tid; f < F; f += NT)
+  {
+    int hi = f / STRIDE;
+    int lo = f - hi * STRIDE;
+    fiber_apply<N, EXACT, T>(buf + hi * STRIDE * N + lo, STRIDE, am);
+  }
+  __syncthreads();
+  if constexpr (K + 2 < D)
+    slab_stage<T, D, N, NT, EXACT, K + 1>(buf, a, b, tid);
+}
+
+template<typename T, int D, int N, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
+{
+  constexpr int SL = cpow(N, D - 1);
+  constexpr int RS = (SL + NT - 1) / NT;
+  static_assert((SL * sizeof(T)) % 16 == 0, "slab must be whole 16-byte chunks");
+  __shared__ __attribute__((aligned(16))) T sb0[SL];
+  __shared__ __attribute__((aligned(16))) T sb1[SL];
+  const int tid = threadIdx.x;
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+
+  using C = typename Acc<T>::type;
+  C acc[N][RS];
+  T *target = nullptr;
+
+  // slab sequence: (entry p, j1) flattened as t = (p - p0)*N + j1
+  const int nslab = (p1 - p0) * N;
+  {
+    const T *xp = x_of(a, entry_at(a, p0));
+    slab_async_load<T, SL, NT>(sb0, xp, tid);
+  }
+
+  for (int t = 0; t < nslab; ++t)
+  {
+    const int p = p0 + t / N;
+    const int j1 = t - (t / N) * N;
+    const int b = entry_at(a, p);
+    T *cur = (t & 1) ? sb1 : sb0;
+    T *nxt = (t & 1) ? sb0 : sb1;
+
+    if (j1 == 0)
+    {
+      T *yb = y_of(a, b);
+      if (yb != target)
+      {
+        if (target)
+        {
+#pragma unroll
+          for (int i1 = 0; i1 < N; i1++)
+#pragma unroll
+            for (int r = 0; r < RS; r++)
+            {
+              int q = tid + NT * r;
+              if (SL % NT == 0 || q < SL)
+                flush_add(target + i1 * SL + q, static_cast<T>(acc[i1][r]),
+                          a.owned);
+            }
+        }
+#pragma unroll
+        for (int i1 = 0; i1 < N; i1++)
+#pragma unroll
+          for (int r = 0; r < RS; r++)
+            acc[i1][r] = 0;
+        target = yb;
+      }
+    }
+
+    // the current slab has landed for every wave; nxt is free again
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (t + 1 < nslab)
+    {
+      const int pn = p0 + (t + 1) / N;
+      const int jn = (t + 1) - ((t + 1) / N) * N;
+      const T *xp = x_of(a, entry_at(a, pn));
+      slab_async_load<T, SL, NT>(nxt, xp + (size_t)jn * SL, tid);
+    }
+
+    slab_stage<T, D, N, NT, EXACT, 0>(cur, a, b, tid);
+
+    const T *A1 = a_of(a, b, 0, D);
+    C a1[N];
+#pragma unroll
+    for (int i1 = 0; i1 < N; i1++)
+      a1[i1] = cld(A1 + (i1 + j1 * a.lda));
+#pragma unroll
+    for (int r = 0; r < RS; r++)
+    {
+      int q = tid + NT * r;
+      if (SL % NT == 0 || q < SL)
+      {
+        C z = cur[q];
+#pragma unroll
+        for (int i1 = 0; i1 < N; i1++)
+          acc[i1][r] = madd<EXACT>(acc[i1][r], a1[i1], z);
+      }
+    }
+  }
+#pragma unroll
+  for (int i1 = 0; i1 < N; i1++)
+#pragma unroll
+    for (int r = 0; r < RS; r++)
+    {
+      int q = tid + NT * r;
+      if (SL % NT == 0 || q < SL)
+        flush_add(target + i1 * SL + q, static_cast<T>(acc[i1][r]), a.owned);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kron_global: any d <= 6 and any (rectangular) m_k x n_k with
+// n_k <= KM_GLOBAL_MAX_N; one workgroup per entry; stages go through the caller's workspace as in the reference
+// (kronm_backward.hpp:92-139).  Square shapes update one buffer in place;
+// rectangular shapes ping-pong between the slot's two buffers.  The final
+// stage adds into Y with atomics, or with a plain read-modify-write when the
+// workgroup owns the output (the barrier after every stage orders the
+// entries of one output inside the workgroup).
+// ---------------------------------------------------------------------------
+template<typename T, bool EXACT>
+__global__ __launch_bounds__(256) void kron_global(EntryArgs<T> a, Shape sh,
+                                                   int fwd)
+{
+  using C = typename Acc<T>::type;
+  const int tid = threadIdx.x;
+  const int d = sh.d;
+  int p0, p1;
+  wg_range(a, p0, p1);
+  bool square = true;
+  for (int k = 0; k < d; k++)
+    square = square && (sh.m[k] == sh.n[k]);
+  T *w0 = a.pW ? cld(a.pW + a.ws_mul * blockIdx.x)
+               : a.W + blockIdx.x * a.w_stride;
+  T *w1 = square ? w0
+                 : (a.pW ? cld(a.pW + (a.ws_mul * blockIdx.x + 1))
+                         : a.W + blockIdx.x * a.w_stride + a.w_half);
+
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    const T *src = x_of(a, b);
+    T *yb = y_of(a, b);
+    // current tensor extents: mode k has extent ext[k]
+    int ext[6];
+    for (int k = 0; k < d; k++)
+      ext[k] = sh.n[k];
+    T *bufs[2] = {w0, w1};
+    int which = 0;
+    for (int s = 0; s < d; s++)
+    {
+      const int k = fwd ? s : d - 1 - s;
+      const int nk = sh.n[k], mk = sh.m[k];
+      int inner = 1;
+      for (int q = k + 1; q < d; q++)
+        inner *= ext[q];
+      int outer = 1;
+      for (int q = 0; q < k; q++)
+        outer *= ext[q];
+      const T *Ak = a_of(a, b, k, d);
+      const int ld = a.lda ? a.lda : mk;
+      const bool last = (s == d - 1);
+      T *dst = bufs[which];
+      const long long fibers = (long long)outer * inner;
+      for (long long f = tid; f < fibers; f += blockDim.x)
+      {
+        long long hi = f / inner, lo = f - hi * inner;
+        const T *xin = src + hi * inner * nk + lo;
+        C x[KM_GLOBAL_MAX_N];
+        for (int j = 0; j < nk; j++)
+          x[j] = static_cast<C>(gld(xin + (long long)j * inner));
+        for (int i = 0; i < mk; i++)
+        {
+          C acc = 0;
+          for (int j = 0; j < nk; j++)
+            acc = madd<EXACT>(acc, static_cast<C>(cld(Ak + (i + j * ld))), x[j]);
+          long long o = hi * inner * mk + lo + (long long)i * inner;
+          if (last)
+            flush_add(yb + o, static_cast<T>(acc), a.owned);
+          else
+            gst(dst + o, static_cast<T>(acc));
+        }
+      }
+      __syncthreads();
+      ext[k] = mk;
+      src = dst;
+      if (!square)
+        which ^= 1;
+    }
+  }
+}
+
+} // namespace km

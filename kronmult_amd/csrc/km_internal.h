@@ -1,0 +1,62 @@
+// km_internal.h -- host-side kernel registry shared by the instantiation
+// units (km_lds_d*.hip, km_slab.hip, km_global.hip) and the C ABI (km_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "km_device.h"
+
+namespace km
+{
+
+enum Family
+{
+  FAM_LDS = 0,
+  FAM_SLAB = 1,
+  FAM_GLOBAL = 2
+};
+
+struct KernelInfo
+{
+  const void *fn = nullptr; // __global__ function
+  int nt = 0;               // threads per workgroup
+  int family = FAM_LDS;
+};
+
+// threads per workgroup of kron_lds for a tensor of s elements
+constexpr int lds_threads(int s) { return s <= 1024 ? 64 : (s <= 4096 ? 256 : 512); }
+// largest tensor kron_lds keeps in LDS (doubles): 150 KiB of the 160 KiB
+constexpr int lds_max_elems(int elem_bytes) { return 150 * 1024 / elem_bytes; }
+
+// kron_lds instantiation for (d, n) if one was built (n <= 8, n^d fits LDS)
+template<typename T>
+bool lds_kernel(int d, int n, bool exact, bool fwd, KernelInfo *out);
+// per-d units
+template<typename T, int D>
+bool lds_kernel_d(int n, bool exact, bool fwd, KernelInfo *out);
+
+// kron_slab instantiation for (d, n) if one was built
+template<typename T>
+bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
+
+// kron_global (always available)
+template<typename T>
+KernelInfo global_kernel(bool exact);
+
+} // namespace km
+
+// Instantiation helper for one (T, D, N): fills *out with the variant asked for.
+#define KM_LDS_CASE(T, D, N)                                                   \
+  case N:                                                                      \
+  {                                                                            \
+    constexpr int NT_ = ::km::lds_threads(::km::cpow(N, D));                   \
+    if (exact && fwd)                                                          \
+      out->fn = (const void *)&::km::kron_lds<T, D, N, NT_, true, true>;       \
+    else if (exact)                                                            \
+      out->fn = (const void *)&::km::kron_lds<T, D, N, NT_, true, false>;      \
+    else                                                                       \
+      out->fn = (const void *)&::km::kron_lds<T, D, N, NT_, false, false>;     \
+    out->nt = NT_;                                                             \
+    out->family = ::km::FAM_LDS;                                               \
+    return true;                                                               \
+  }

@@ -1,0 +1,212 @@
+// ref_driver.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// Builds the reference's own CPU/OpenMP code path, straight from the reference
+// headers where they lie (include path given on the command line by
+// oracle/Makefile; nothing is copied), into oracle/_ref/libkronref_*.so.  The
+// extern "C" functions below only instantiate the reference's templates for
+// the CPU build and forward their arguments; they have the same signatures as
+// the oracle's restatement (kron_oracle.c) so the two can be compared call for
+// call.  The library is used to write golden vectors and, on machines where it
+// was built, as the "reference" CPU baseline.
+//
+// Reference entry points instantiated here:
+//   kronmult{1..6}_batched<T>   kronmult6_batched.hpp:12-75
+//   kronmult{1..6}_pbatched<T>  kronmult6_pbatched.hpp:12-67
+//   kronmult{1..6}_xbatched<T>  kronmult6_xbatched.hpp:10-31 -> kronmult_xbatched.hpp:17-162
+//   kronmult{1..6}_vbatched<T>  kronmult6_vbatched.hpp:10-34 -> kronmult_vbatched.hpp:17-228
+#include <cstddef>
+#include <cstdio>
+
+#include "kronmult1_batched.hpp"
+#include "kronmult2_batched.hpp"
+#include "kronmult3_batched.hpp"
+#include "kronmult4_batched.hpp"
+#include "kronmult5_batched.hpp"
+#include "kronmult6_batched.hpp"
+
+#include "kronmult1_pbatched.hpp"
+#include "kronmult2_pbatched.hpp"
+#include "kronmult3_pbatched.hpp"
+#include "kronmult4_pbatched.hpp"
+#include "kronmult5_pbatched.hpp"
+#include "kronmult6_pbatched.hpp"
+
+#include "kronmult1_xbatched.hpp"
+#include "kronmult2_xbatched.hpp"
+#include "kronmult3_xbatched.hpp"
+#include "kronmult4_xbatched.hpp"
+#include "kronmult5_xbatched.hpp"
+#include "kronmult6_xbatched.hpp"
+
+#include "kronmult1_vbatched.hpp"
+#include "kronmult2_vbatched.hpp"
+#include "kronmult3_vbatched.hpp"
+#include "kronmult4_vbatched.hpp"
+#include "kronmult5_vbatched.hpp"
+#include "kronmult6_vbatched.hpp"
+
+namespace
+{
+
+template<typename T>
+int batched(int nd, int n, T const *A, T *X, T *Y, T *W, int batch)
+{
+  switch (nd)
+  {
+  case 1: kronmult1_batched<T>(n, A, X, Y, W, batch); return 0;
+  case 2: kronmult2_batched<T>(n, A, X, Y, W, batch); return 0;
+  case 3: kronmult3_batched<T>(n, A, X, Y, W, batch); return 0;
+  case 4: kronmult4_batched<T>(n, A, X, Y, W, batch); return 0;
+  case 5: kronmult5_batched<T>(n, A, X, Y, W, batch); return 0;
+  case 6: kronmult6_batched<T>(n, A, X, Y, W, batch); return 0;
+  }
+  return -1;
+}
+
+template<typename T>
+int pbatched(int nd, int n, T const *A, T **pX, T **pY, T **pW, int batch)
+{
+  switch (nd)
+  {
+  case 1: kronmult1_pbatched<T>(n, A, pX, pY, pW, batch); return 0;
+  case 2: kronmult2_pbatched<T>(n, A, pX, pY, pW, batch); return 0;
+  case 3: kronmult3_pbatched<T>(n, A, pX, pY, pW, batch); return 0;
+  case 4: kronmult4_pbatched<T>(n, A, pX, pY, pW, batch); return 0;
+  case 5: kronmult5_pbatched<T>(n, A, pX, pY, pW, batch); return 0;
+  case 6: kronmult6_pbatched<T>(n, A, pX, pY, pW, batch); return 0;
+  }
+  return -1;
+}
+
+template<typename T>
+int xbatched(int nd, int n, T const *const *A, int lda, T **pX, T **pY,
+             T **pW, int batch, int sub)
+{
+  switch (nd)
+  {
+  case 1: kronmult1_xbatched<T>(n, A, lda, pX, pY, pW, batch, sub); return 0;
+  case 2: kronmult2_xbatched<T>(n, A, lda, pX, pY, pW, batch, sub); return 0;
+  case 3: kronmult3_xbatched<T>(n, A, lda, pX, pY, pW, batch, sub); return 0;
+  case 4: kronmult4_xbatched<T>(n, A, lda, pX, pY, pW, batch, sub); return 0;
+  case 5: kronmult5_xbatched<T>(n, A, lda, pX, pY, pW, batch, sub); return 0;
+  case 6: kronmult6_xbatched<T>(n, A, lda, pX, pY, pW, batch, sub); return 0;
+  }
+  return -1;
+}
+
+// The reference forces a sub-batch of 1 when Wcap is too small and then
+// writes past W (kronmult_vbatched.hpp:131-135); refuse that case here so the
+// driver never runs the overflow.
+template<typename T>
+int vbatched(int nd, int const *m, int const *n, T const *const *A, T **pX,
+             T **pY, T *W, size_t Wcap, int batch)
+{
+  int mm[6] = {1, 1, 1, 1, 1, 1}, nn[6] = {1, 1, 1, 1, 1, 1};
+  for (int k = 0; k < nd; k++)
+  {
+    mm[k] = m[k];
+    nn[k] = n[k];
+  }
+  size_t sx = 1, sw = 0;
+  for (int k = 0; k < 6; k++)
+    sx *= nn[k];
+  for (int L = 1; L < 6; L++)
+  {
+    size_t s = 1;
+    for (int k = 0; k < L; k++)
+      s *= mm[k];
+    for (int k = L; k < 6; k++)
+      s *= nn[k];
+    sw = (s > sw) ? s : sw;
+  }
+  size_t sxw = (sx > sw) ? sx : sw;
+  if (Wcap < 2 * sxw * sizeof(T))
+    return -1;
+  switch (nd)
+  {
+  case 1:
+    kronmult1_vbatched<T>(mm[0], nn[0], A, pX, pY, W, Wcap, batch);
+    return 0;
+  case 2:
+    kronmult2_vbatched<T>(mm[0], nn[0], mm[1], nn[1], A, pX, pY, W, Wcap,
+                          batch);
+    return 0;
+  case 3:
+    kronmult3_vbatched<T>(mm[0], nn[0], mm[1], nn[1], mm[2], nn[2], A, pX, pY,
+                          W, Wcap, batch);
+    return 0;
+  case 4:
+    kronmult4_vbatched<T>(mm[0], nn[0], mm[1], nn[1], mm[2], nn[2], mm[3],
+                          nn[3], A, pX, pY, W, Wcap, batch);
+    return 0;
+  case 5:
+    kronmult5_vbatched<T>(mm[0], nn[0], mm[1], nn[1], mm[2], nn[2], mm[3],
+                          nn[3], mm[4], nn[4], A, pX, pY, W, Wcap, batch);
+    return 0;
+  case 6:
+    kronmult6_vbatched<T>(mm[0], nn[0], mm[1], nn[1], mm[2], nn[2], mm[3],
+                          nn[3], mm[4], nn[4], mm[5], nn[5], A, pX, pY, W,
+                          Wcap, batch);
+    return 0;
+  }
+  return -1;
+}
+
+} // namespace
+
+extern "C"
+{
+#ifdef USE_ALG993
+  int ref_variant(void) { return 3; }
+#else
+  int ref_variant(void) { return 0; }
+#endif
+
+  int ref_batched_f64(int nd, int n, double const *A, double *X, double *Y,
+                      double *W, int batch)
+  {
+    return batched<double>(nd, n, A, X, Y, W, batch);
+  }
+  int ref_batched_f32(int nd, int n, float const *A, float *X, float *Y,
+                      float *W, int batch)
+  {
+    return batched<float>(nd, n, A, X, Y, W, batch);
+  }
+  int ref_pbatched_f64(int nd, int n, double const *A, double **pX,
+                       double **pY, double **pW, int batch)
+  {
+    return pbatched<double>(nd, n, A, pX, pY, pW, batch);
+  }
+  int ref_pbatched_f32(int nd, int n, float const *A, float **pX, float **pY,
+                       float **pW, int batch)
+  {
+    return pbatched<float>(nd, n, A, pX, pY, pW, batch);
+  }
+  int ref_xbatched_f64(int nd, int n, double const *const *A, int lda,
+                       double **pX, double **pY, double **pW, int batch,
+                       int sub)
+  {
+    return xbatched<double>(nd, n, A, lda, pX, pY, pW, batch, sub);
+  }
+  int ref_xbatched_f32(int nd, int n, float const *const *A, int lda,
+                       float **pX, float **pY, float **pW, int batch, int sub)
+  {
+    return xbatched<float>(nd, n, A, lda, pX, pY, pW, batch, sub);
+  }
+  int ref_vbatched_f64(int nd, int const *m, int const *n,
+                       double const *const *A, double **pX, double **pY,
+                       double *W, size_t Wcap, int batch)
+  {
+    return vbatched<double>(nd, m, n, A, pX, pY, W, Wcap, batch);
+  }
+  int ref_vbatched_f32(int nd, int const *m, int const *n,
+                       float const *const *A, float **pX, float **pY, float *W,
+                       size_t Wcap, int batch)
+  {
+    return vbatched<float>(nd, m, n, A, pX, pY, W, Wcap, batch);
+  }
+  double ref_kron_flops(int nd, int const *m, int const *n)
+  {
+    return kron_flops(nd, m, n);
+  }
+}
