@@ -369,6 +369,13 @@ typedef __attribute__((address_space(1))) void *gbl_vptr;
 template<typename T, int SL, int NT>
 __device__ __forceinline__ void slab_async_load(T *dst, const T *src, int tid)
 {
+  if (reinterpret_cast<uintptr_t>(src) & 15)
+  {
+    // misaligned vector: ordinary element loads (still coalesced)
+    for (int q = tid; q < SL; q += NT)
+      dst[q] = gld(src + q);
+    return;
+  }
   constexpr int CH = SL * (int)sizeof(T) / 16; // 16-byte chunks
   constexpr int IT = (CH + NT - 1) / NT;
   const int wave = tid >> 6;
