@@ -1,0 +1,212 @@
+"""Benchmark: d=6 batched Kronecker mat-vec on one MI355X.
+
+Workload (the plain-FP64 GPU configuration): kronmult6_pbatched, FP64,
+d = 6, n = 6 (vectors of 6^6 = 46656 doubles), 4096 entries, every entry with
+its own six 6x6 A ~ U[-1,1) and its own X ~ U[-1,1); outputs overlap 4:1,
+pY[k] = Y + (k/4)*n^d.  One step = one kronmult6_pbatched call over the whole
+batch, inputs resident in HBM.
+
+Printed: one JSON line with GFLOP/s (reference flop convention 2*d*n^(d+1) per
+entry, test_kronmult6_batched.cpp:270), the HBM roofline of the kernel
+(algorithmic bytes = X + A read once, Y read and written once), and the
+reference's CPU path timed on this machine's host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode atomic|plan]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+D, N, BATCH, OVERLAP = 6, 6, 4096, 4
+SEED = 20261203
+
+
+def flops_per_call(d, n, batch):
+    return 2.0 * d * n ** (d + 1) * batch
+
+
+def algorithmic_bytes(d, n, batch, overlap, elem=8):
+    s = n ** d
+    n_y = batch // overlap
+    return elem * (batch * s + batch * d * n * n + 2 * n_y * s)
+
+
+def parse_traffic(path):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary: lines
+    'traffic_bytes_per_launch: <int>'."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1].strip()))
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(A, X, yidx, n_y, reps=3):
+    """The reference's own CPU/OpenMP path (oracle/_ref, default recursive
+    build) when it was built, else the oracle restatement; timed on the host
+    cores over the full workload (median of `reps`)."""
+    import numpy as np
+
+    from oracle import oracle as O
+    use_ref = "recursive" if O.ref_available("recursive") else None
+    threads = O.num_threads()
+    d, n = D, N
+    batch = A.shape[0]
+    s = n ** d
+    Y = np.zeros((n_y, s))
+    times = []
+    for _ in range(reps):
+        # the reference uses X as scratch: every rep gets fresh copies, and
+        # only the library call itself is timed
+        if use_ref:
+            O.pbatched(A, X, Y, yidx, use_ref=use_ref, timings=times)
+        else:
+            O.pbatched(A, X, Y, yidx, variant=O.RECURSIVE, timings=times)
+    t = statistics.median(times)
+    return {
+        "value": round(flops_per_call(d, n, batch) / t / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": threads,
+        "kind": "reference" if use_ref else "port",
+        "sample": f"full workload ({batch} entries, d={d}, n={n}, 4:1 overlap), median of {reps} "
+                  f"calls of kronmult6_pbatched (default recursive build), OpenMP threads={threads}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", choices=["atomic", "plan"], default="atomic")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the batched kernel on one "
+              "GPU's share); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import kronmult_amd as km
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    d, n, batch = D, N, BATCH
+    s = n ** d
+    n_y = batch // OVERLAP
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(SEED)
+    A = torch.rand((batch, d, n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1
+    X = torch.rand((batch, s), generator=g, device=dev, dtype=torch.float64) * 2 - 1
+    Y = torch.zeros((n_y, s), device=dev, dtype=torch.float64)
+    yidx = np.arange(batch) // OVERLAP
+    pX = km.pointer_array(X, np.arange(batch))
+    pY = km.pointer_array(Y, yidx)
+
+    plan = None
+    opt = None
+    if args.mode == "plan":
+        plan = km.Plan(d, n, 8, pY, batch)
+        opt = km.options(plan=plan)
+
+    fam, nt = km.query_kernel(d, n, 8, opt)
+    kernel_name = {0: "kron_lds", 1: "kron_slab", 2: "kron_global"}[fam]
+
+    stream = torch.cuda.current_stream()
+
+    def step():
+        km.kronmult_pbatched(d, n, A, pX, pY, None, batch, stream=stream, opt=opt)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / args.steps
+
+    fl = flops_per_call(d, n, batch)
+    by = algorithmic_bytes(d, n, batch, OVERLAP)
+    gflops = fl / (ms_per_step * 1e-3) / 1e9
+    gbs = by / (ms_per_step * 1e-3) / 1e9
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"{kernel_name}_d6n6_b4096_traffic.txt"))
+
+    cpu = None
+    if not args.no_cpu:
+        # same seeded inputs, host copies
+        cpu = cpu_baseline(A.cpu().numpy(), X.cpu().numpy(), yidx, n_y)
+
+    out = {
+        "metric": "d=6 batched kronmult GFLOP/s + % HBM roofline; mixed vs FP64 at rel err <=1e-10",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U[-1,1) A and X on device)",
+        "config": {
+            "workload": "kronmult6_pbatched FP64, d=6, n=6, batch=4096, 4:1 output overlap "
+                        "(pY[k] = Y + (k/4)*n^d)",
+            "d": d, "n": n, "batch": batch, "overlap": OVERLAP,
+            "reduction": "plan (deterministic, owned outputs)" if plan else "atomic (drop-in)",
+            "kernel": f"{kernel_name} ({nt} threads/workgroup)",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "wall_s": round(wall, 4),
+    }
+    print(json.dumps(out))
+    if plan is not None:
+        plan.close()
+
+
+if __name__ == "__main__":
+    main()

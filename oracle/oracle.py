@@ -29,6 +29,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import time
 from typing import Optional, Sequence
 
 import numpy as np
@@ -164,9 +165,11 @@ def batched(A: np.ndarray, X: np.ndarray, Y: np.ndarray, variant: int = RECURSIV
 
 def pbatched(A: np.ndarray, X: np.ndarray, Y: np.ndarray, yidx: Sequence[int],
              xidx: Optional[Sequence[int]] = None, variant: int = RECURSIVE,
-             use_ref: Optional[str] = None) -> np.ndarray:
+             use_ref: Optional[str] = None, timings: Optional[list] = None) -> np.ndarray:
     """kronmultN_pbatched: entry b reads X[xidx[b]] and accumulates into
-    Y[yidx[b]].  Every entry gets a private X copy (the reference clobbers X)."""
+    Y[yidx[b]].  Every entry gets a private X copy (the reference clobbers X).
+    When ``timings`` is a list, the wall time of the library call alone
+    (without the copies) is appended to it."""
     batch, d, n, _ = A.shape
     A = np.ascontiguousarray(A)
     xidx = np.arange(batch) if xidx is None else np.asarray(xidx)
@@ -177,11 +180,14 @@ def pbatched(A: np.ndarray, X: np.ndarray, Y: np.ndarray, yidx: Sequence[int],
     pY = _rowptrs(Yo, yidx)
     pW = _rowptrs(W, np.arange(batch))
     suf = _suffix(A.dtype)
+    t0 = time.perf_counter()
     if use_ref:
         rc = getattr(ref_lib(use_ref), f"ref_pbatched_{suf}")(d, n, _ptr(A), _ptr(pX), _ptr(pY), _ptr(pW), batch)
         assert rc == 0
     else:
         getattr(lib(), f"oracle_pbatched_{suf}")(d, n, _ptr(A), _ptr(pX), _ptr(pY), _ptr(pW), batch, variant)
+    if timings is not None:
+        timings.append(time.perf_counter() - t0)
     return Yo
 
 
