@@ -22,7 +22,8 @@ import os
 from typing import Optional, Sequence
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkronmult_amd.so")
+# KRONMULT_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("KRONMULT_AMD_LIB", os.path.join(HERE, "libkronmult_amd.so"))
 
 # km_status
 SUCCESS = 0

@@ -32,6 +32,16 @@ namespace km
 // largest mode extent n_k accepted by the workspace-staged kernel
 #define KM_GLOBAL_MAX_N 32
 
+// fiber-loop unroll factors of the LDS stage passes (tuning knobs)
+#ifndef KM_LDS_UNROLL
+#define KM_LDS_UNROLL 2
+#endif
+#ifndef KM_SLAB_UNROLL
+#define KM_SLAB_UNROLL 1
+#endif
+constexpr int kLdsUnroll = KM_LDS_UNROLL;
+constexpr int kSlabUnroll = KM_SLAB_UNROLL;
+
 // ---------------------------------------------------------------------------
 // arguments
 // ---------------------------------------------------------------------------
@@ -237,6 +247,74 @@ __device__ __forceinline__ void flush_add(T *dst, T v, int owned)
     gatomic_add(dst, v);
 }
 
+// dst[tid + NT*r] += acc[r] for r < R (positions below S).  Owned outputs are
+// read-modified-written with all loads of a chunk issued before its stores,
+// so the memory latency is paid once per chunk rather than once per element;
+// shared outputs use one hardware FP64 atomic add per element.
+template<int R, int NT, int S, int CHUNK, typename T, typename C>
+__device__ __forceinline__ void flush_run(T *dst, const C (&acc)[R], int tid,
+                                          int owned)
+{
+  if (owned)
+  {
+#pragma unroll
+    for (int r0 = 0; r0 < R; r0 += CHUNK)
+    {
+      T yv[CHUNK];
+#pragma unroll
+      for (int c = 0; c < CHUNK; c++)
+      {
+        const int r = r0 + c, q = tid + NT * r;
+        if (r < R && (S % NT == 0 || q < S))
+          yv[c] = gld(dst + q);
+      }
+#pragma unroll
+      for (int c = 0; c < CHUNK; c++)
+      {
+        const int r = r0 + c, q = tid + NT * r;
+        if (r < R && (S % NT == 0 || q < S))
+          gst(dst + q, yv[c] + static_cast<T>(acc[r]));
+      }
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (S % NT == 0 || q < S)
+        gatomic_add(dst + q, static_cast<T>(acc[r]));
+    }
+  }
+}
+
+// Pull the next entry's A matrices into L2 with one vector load per 128-byte
+// line; the value is consumed (keep_alive) only after the current entry, so
+// the later scalar loads of that entry hit L2 instead of HBM.
+template<int D, int N, typename T>
+__device__ __forceinline__ T prefetch_a(const EntryArgs<T> &a, int b, int tid)
+{
+  constexpr int PER_LINE = 128 / (int)sizeof(T);
+  T v = 0;
+  const int span = (N - 1) * a.lda + N; // elements of one matrix
+  const int lines = (span + PER_LINE - 1) / PER_LINE;
+#pragma unroll
+  for (int k = 0; k < D; k++)
+  {
+    const T *Ak = a_of(a, b, k, D);
+    if (tid < lines)
+      v += gld(Ak + min(tid * PER_LINE, span - 1));
+  }
+  return v;
+}
+
+template<typename T>
+__device__ __forceinline__ void keep_alive(T v)
+{
+  asm volatile("" ::"v"(v));
+}
+
 // ---------------------------------------------------------------------------
 // kron_lds: whole tensor in LDS, one workgroup per entry at a time.
 //   FWD = false: modes contracted d..1 (Alg. 993 backward order,
@@ -255,7 +333,7 @@ __device__ __forceinline__ void lds_stage(T *buf, const EntryArgs<T> &a, int b,
   constexpr int STRIDE = cpow(N, D - 1 - MODE);
   C am[N][N];
   load_a<N>(am, a_of(a, b, MODE, D), a.lda);
-#pragma unroll 2
+#pragma unroll kLdsUnroll
   for (int f = tid; f < F; f += NT)
   {
     int hi = f / STRIDE;
@@ -317,18 +395,13 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
           xr[r] = gld(xp + q);
       }
     }
+    T pf = 0;
+    if (p + 1 < p1)
+      pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
     if (yb != target)
     {
       if (target)
-      {
-#pragma unroll
-        for (int r = 0; r < R; r++)
-        {
-          int q = tid + NT * r;
-          if (S % NT == 0 || q < S)
-            flush_add(target + q, acc[r], a.owned);
-        }
-      }
+        flush_run<R, NT, S, 8>(target, acc, tid, a.owned);
 #pragma unroll
       for (int r = 0; r < R; r++)
         acc[r] = 0;
@@ -343,14 +416,9 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
       if (S % NT == 0 || q < S)
         acc[r] = acc[r] + buf[q];
     }
+    keep_alive(pf);
   }
-#pragma unroll
-  for (int r = 0; r < R; r++)
-  {
-    int q = tid + NT * r;
-    if (S % NT == 0 || q < S)
-      flush_add(target + q, acc[r], a.owned);
-  }
+  flush_run<R, NT, S, 8>(target, acc, tid, a.owned);
 }
 
 // ---------------------------------------------------------------------------
@@ -409,7 +477,7 @@ __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
   constexpr int STRIDE = cpow(N, D - 1 - MODE);
   C am[N][N];
   load_a<N>(am, a_of(a, b, MODE, D), a.lda);
-#pragma unroll 1
+#pragma unroll kSlabUnroll
   for (int f = tid; f < F; f += NT)
   {
     int hi = f / STRIDE;
@@ -463,14 +531,7 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
         {
 #pragma unroll
           for (int i1 = 0; i1 < N; i1++)
-#pragma unroll
-            for (int r = 0; r < RS; r++)
-            {
-              int q = tid + NT * r;
-              if (SL % NT == 0 || q < SL)
-                flush_add(target + i1 * SL + q, static_cast<T>(acc[i1][r]),
-                          a.owned);
-            }
+            flush_run<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned);
         }
 #pragma unroll
         for (int i1 = 0; i1 < N; i1++)
@@ -514,13 +575,7 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
   }
 #pragma unroll
   for (int i1 = 0; i1 < N; i1++)
-#pragma unroll
-    for (int r = 0; r < RS; r++)
-    {
-      int q = tid + NT * r;
-      if (SL % NT == 0 || q < SL)
-        flush_add(target + i1 * SL + q, static_cast<T>(acc[i1][r]), a.owned);
-    }
+    flush_run<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned);
 }
 
 // ---------------------------------------------------------------------------
