@@ -172,7 +172,9 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
 
 /* ---------------------------------------------------------------------------
  * Introspection: which kernel family a square (d, n) call would use.
- *   *family: 0 = LDS-resident, 1 = slab-streamed, 2 = workspace-staged.
+ *   *family: 0 = LDS-resident, 1 = slab-streamed, 2 = workspace-staged
+ *            (generic), 3 = two-pass (sub-slabs in LDS, then the two
+ *            leading modes in registers; uses W / pW).
  *   *threads: workgroup size.
  * ------------------------------------------------------------------------ */
 km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,

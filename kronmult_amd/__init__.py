@@ -46,7 +46,7 @@ FLAG_EXACT = 0x1
 FLAG_FORCE_GLOBAL = 0x2
 
 # kernel families reported by query_kernel
-FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL = 0, 1, 2
+FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL, FAMILY_TWOPASS = 0, 1, 2, 3
 
 # every symbol the C ABI exports (include/kronmult_amd.h)
 C_ABI_SYMBOLS = (

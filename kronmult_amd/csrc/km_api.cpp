@@ -166,6 +166,82 @@ struct Call
   int elem_bytes = sizeof(T);
 };
 
+// Two-pass family: pass 1 over the sub-slabs of a chunk of entries, pass 2
+// over the chunk's entries.  Chunks are as large as the workspace allows:
+// all entries when every entry brings its own workspace (batched,
+// pbatched), else one slot per entry of the chunk (xbatched, vbatched).
+template<typename T>
+km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
+                         const KernelInfo &k2, hipStream_t stream)
+{
+  const int batch = c.a.batch;
+  const int d = c.sh.d, n = c.sh.n[0];
+  const int nn = n * n;
+  const long long M = ipow(n, d - 2);
+  const int cus = device_cus();
+  if (cus <= 0)
+    return KM_ERROR_HIP;
+  const long long chunk_entries = c.a.slot_by_entry ? batch : c.slots;
+  if (chunk_entries < 1)
+    return KM_ERROR_INSUFFICIENT_WORKSPACE;
+  if (r.plan)
+  {
+    const km_plan p = r.plan;
+    if (p->batch != batch || p->elem_bytes != (int)sizeof(T) || p->d != d ||
+        p->n != n)
+      return KM_ERROR_PLAN_MISMATCH;
+    if (chunk_entries < batch)
+      return KM_ERROR_NOT_SUPPORTED; // a plan needs every entry's workspace
+    c.a.order = p->order;
+  }
+  const int occ1 = occupancy(k1.fn, k1.nt);
+  const int occ2 = occupancy(k2.fn, k2.nt);
+  const int gy = (int)((M + k2.nt - 1) / k2.nt);
+  for (long long e0 = 0; e0 < batch; e0 += chunk_entries)
+  {
+    const int cnt = (int)std::min<long long>(chunk_entries, batch - e0);
+    int e0i = (int)e0;
+    int e1i = (int)(e0 + cnt);
+    // pass 1
+    int nsub = cnt * nn;
+    long long g1 = std::max(1LL, std::min<long long>((long long)cus * occ1, nsub));
+    c.a.chunk = (int)((nsub + g1 - 1) / g1);
+    c.a.ranges = nullptr;
+    int grid1 = (nsub + c.a.chunk - 1) / c.a.chunk;
+    void *args1[] = {&c.a, &e0i, &nsub};
+    if (hipLaunchKernel(k1.fn, dim3(grid1), dim3(k1.nt), args1, 0, stream) !=
+        hipSuccess)
+    {
+      (void)hipGetLastError();
+      return KM_ERROR_HIP;
+    }
+    // pass 2
+    int gx;
+    if (r.plan)
+    {
+      c.a.ranges = r.plan->ranges;
+      c.a.owned = 1;
+      gx = r.plan->ngroups;
+    }
+    else
+    {
+      long long want = std::max(1LL, (long long)cus * occ2 * 2 / gy);
+      long long g = std::min<long long>(want, cnt);
+      c.a.chunk = (int)((cnt + g - 1) / g);
+      c.a.owned = 0;
+      gx = (cnt + c.a.chunk - 1) / c.a.chunk;
+    }
+    void *args2[] = {&c.a, &e0i, &e1i};
+    if (hipLaunchKernel(k2.fn, dim3(gx, gy), dim3(k2.nt), args2, 0, stream) !=
+        hipSuccess)
+    {
+      (void)hipGetLastError();
+      return KM_ERROR_HIP;
+    }
+  }
+  return KM_SUCCESS;
+}
+
 template<typename T>
 km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
 {
@@ -174,6 +250,16 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
     return KM_SUCCESS;
   KernelInfo k;
   bool onchip = c.square && pick_onchip<T>(c.sh.d, c.sh.n[0], r, &k);
+  if (!onchip && c.square && !r.force_global && !(r.fwd && r.exact))
+  {
+    KernelInfo k1, k2;
+    if (km::twopass_kernels<T>(c.sh.d, c.sh.n[0], r.exact, &k1, &k2))
+    {
+      if (c.slots < 1)
+        return KM_ERROR_INSUFFICIENT_WORKSPACE;
+      return launch_twopass(c, r, k1, k2, stream);
+    }
+  }
   if (!onchip)
   {
     for (int i = 0; i < c.sh.d; i++)
@@ -284,6 +370,7 @@ km_status batched(int d, int n, const T *A, const T *X, T *Y, T *W, int batch,
   c.a.w_stride = xs;
   c.a.w_half = 0;
   c.a.ws_mul = 1;
+  c.a.slot_by_entry = 1;
   c.a.batch = batch;
   c.slots = W ? batch : 0;
   return launch(c, r, (hipStream_t)stream);
@@ -313,6 +400,7 @@ km_status pbatched(int d, int n, const T *A, T *const *pX, T *const *pY,
   c.a.pY = pY;
   c.a.pW = pW;
   c.a.ws_mul = 1;
+  c.a.slot_by_entry = 1;
   c.a.batch = batch;
   c.slots = pW ? batch : 0;
   return launch(c, r, (hipStream_t)stream);
@@ -541,10 +629,22 @@ extern "C"
     KernelInfo k;
     bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
                                     : pick_onchip<float>(d, n, r, &k);
-    if (!onchip)
+    long long G;
+    KernelInfo k1, k2;
+    bool twopass = !onchip && ((elem_bytes == 8)
+                                   ? km::twopass_kernels<double>(d, n, false, &k1, &k2)
+                                   : km::twopass_kernels<float>(d, n, false, &k1, &k2));
+    if (!onchip && !twopass)
       k = (elem_bytes == 8) ? km::global_kernel<double>(false)
                             : km::global_kernel<float>(false);
-    long long G = (long long)device_cus() * occupancy(k.fn, k.nt);
+    if (twopass)
+    {
+      // pass 2 runs one workgroup per (entry group, block of positions)
+      const long long gy = (ipow(n, d - 2) + k2.nt - 1) / k2.nt;
+      G = std::max(1LL, (long long)device_cus() * occupancy(k2.fn, k2.nt) * 2 / gy);
+    }
+    else
+      G = (long long)device_cus() * occupancy(k.fn, k.nt);
     G = std::max(1LL, std::min<long long>(G, batchCount));
     const long long target = (batchCount + G - 1) / G;
     std::vector<int> ranges;
@@ -623,10 +723,15 @@ extern "C"
     Resolved r;
     if ((s = resolve(opt, d, m, nn, &r)) != KM_SUCCESS)
       return s;
-    KernelInfo k;
+    KernelInfo k, k2;
     bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
                                     : pick_onchip<float>(d, n, r, &k);
-    if (!onchip)
+    bool twopass = false;
+    if (!onchip && !r.force_global && !(r.fwd && r.exact))
+      twopass = (elem_bytes == 8)
+                    ? km::twopass_kernels<double>(d, n, r.exact, &k, &k2)
+                    : km::twopass_kernels<float>(d, n, r.exact, &k, &k2);
+    if (!onchip && !twopass)
       k = (elem_bytes == 8) ? km::global_kernel<double>(r.exact)
                             : km::global_kernel<float>(r.exact);
     if (family)

@@ -73,6 +73,8 @@ struct EntryArgs
   const int *ranges;  // workgroup w walks order positions [ranges[w], ranges[w+1])
   const int *order;   // order position -> entry
   int owned;          // 1: workgroup owns its outputs (no atomics)
+  int slot_by_entry;  // two-pass family: workspace slot = entry index (1) or
+                      // position inside the launched chunk (0)
 };
 
 struct Shape
@@ -576,6 +578,294 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
 #pragma unroll
   for (int i1 = 0; i1 < N; i1++)
     flush_run<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned);
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass family, for shapes whose n^(d-1) slab exceeds LDS (d = 6, n = 7, 8):
+//   pass 1 (kron_sub):  for every entry b and every sub-slab (j1, j2) the
+//     trailing modes d..3 are contracted in LDS and the result
+//     Z_b(j1, j2, :) = kron(A_3..A_d) X_b(j1, j2, :) is stored to the entry's
+//     workspace (the caller's W, as in the reference);
+//   pass 2 (kron_pair): per position m of the trailing modes, the n x n values
+//     Z_b(:, :, m) are contracted over mode 2 and then mode 1 in registers and
+//     added into Y.
+// The mode order is d, d-1, ..., 1: the Alg. 993 backward order.
+// ---------------------------------------------------------------------------
+template<typename T>
+__device__ __forceinline__ T *w_of(const EntryArgs<T> &a, int b, int local)
+{
+  const int s = a.slot_by_entry ? b : local;
+  return a.pW ? cld(a.pW + a.ws_mul * s) : a.W + (long long)s * a.w_stride;
+}
+
+// One mode pass over F fibers of stride STRIDE in LDS: every thread first
+// loads all of its fibers (independent LDS reads in flight), then applies A
+// in blocks of RB rows so that at most RB*N entries of A sit in SGPRs (an
+// 8 x 8 FP64 matrix alone would need 128), then writes the fibers back.
+template<typename T, int N, int NT, bool EXACT, int F, int STRIDE>
+__device__ __forceinline__ void stage_apply(T *buf, const T *Ak, int lda,
+                                            int tid)
+{
+  using C = typename Acc<T>::type;
+  constexpr int FPT = (F + NT - 1) / NT;
+  constexpr int RB = (N <= 6) ? N : (N + 1) / 2;
+  C x[FPT][N];
+  C o[FPT][N];
+#pragma unroll
+  for (int f = 0; f < FPT; f++)
+  {
+    const int fi = tid + NT * f;
+    if (F % NT == 0 || fi < F)
+    {
+      const int hi = fi / STRIDE, lo = fi - (fi / STRIDE) * STRIDE;
+      const T *p = buf + hi * STRIDE * N + lo;
+#pragma unroll
+      for (int j = 0; j < N; j++)
+        x[f][j] = static_cast<C>(p[j * STRIDE]);
+    }
+  }
+#pragma unroll
+  for (int r0 = 0; r0 < N; r0 += RB)
+  {
+    C am[RB][N];
+#pragma unroll
+    for (int j = 0; j < N; j++)
+#pragma unroll
+      for (int i = 0; i < RB; i++)
+        if (r0 + i < N)
+          am[i][j] = static_cast<C>(cld(Ak + (r0 + i + j * lda)));
+#pragma unroll
+    for (int f = 0; f < FPT; f++)
+#pragma unroll
+      for (int i = 0; i < RB; i++)
+        if (r0 + i < N)
+        {
+          C acc = 0;
+          if constexpr (EXACT)
+          {
+#pragma unroll
+            for (int j = 0; j < N; j++)
+              acc = madd<true>(acc, am[i][j], x[f][j]);
+          }
+          else
+          {
+            acc = am[i][0] * x[f][0];
+#pragma unroll
+            for (int j = 1; j < N; j++)
+              acc = madd<false>(acc, am[i][j], x[f][j]);
+          }
+          o[f][r0 + i] = acc;
+        }
+  }
+#pragma unroll
+  for (int f = 0; f < FPT; f++)
+  {
+    const int fi = tid + NT * f;
+    if (F % NT == 0 || fi < F)
+    {
+      const int hi = fi / STRIDE, lo = fi - (fi / STRIDE) * STRIDE;
+      T *p = buf + hi * STRIDE * N + lo;
+#pragma unroll
+      for (int i = 0; i < N; i++)
+        p[i * STRIDE] = static_cast<T>(o[f][i]);
+    }
+  }
+}
+
+// Contract modes MODE, MODE-1, ..., LAST of an LDS tensor that holds modes
+// LAST..D-1 of one entry.
+template<typename T, int D, int N, int NT, bool EXACT, int LAST, int MODE>
+__device__ __forceinline__ void trailing_stages(T *buf, const EntryArgs<T> &a,
+                                                int b, int tid)
+{
+  constexpr int SZ = cpow(N, D - LAST);
+  constexpr int F = SZ / N;
+  constexpr int STRIDE = cpow(N, D - 1 - MODE);
+  stage_apply<T, N, NT, EXACT, F, STRIDE>(buf, a_of(a, b, MODE, D), a.lda, tid);
+  __syncthreads();
+  if constexpr (MODE > LAST)
+    trailing_stages<T, D, N, NT, EXACT, LAST, MODE - 1>(buf, a, b, tid);
+}
+
+// pass 1: sub-entries s in [0, nsub) of the chunk starting at order position
+// e0; sub-entry s is sub-slab s % (N*N) of the entry at position e0 + s/(N*N)
+template<typename T, int D, int N, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void kron_sub(EntryArgs<T> a, int e0, int nsub)
+{
+  constexpr int M = cpow(N, D - 2);
+  constexpr int R = (M + NT - 1) / NT;
+  constexpr int NN = N * N;
+  __shared__ T buf[M];
+  const int tid = threadIdx.x;
+  const int s0 = blockIdx.x * a.chunk;
+  const int s1 = min(nsub, s0 + a.chunk);
+  if (s0 >= s1)
+    return;
+  T xr[R];
+  {
+    const int b = entry_at(a, e0 + s0 / NN);
+    const T *xp = x_of(a, b) + (long long)(s0 % NN) * M;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      int q = tid + NT * r;
+      if (M % NT == 0 || q < M)
+        xr[r] = gld(xp + q);
+    }
+  }
+  for (int s = s0; s < s1; ++s)
+  {
+    const int p = e0 + s / NN;
+    const int js = s % NN;
+    const int b = entry_at(a, p);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      int q = tid + NT * r;
+      if (M % NT == 0 || q < M)
+        buf[q] = xr[r];
+    }
+    if (s + 1 < s1)
+    {
+      const int bn = entry_at(a, e0 + (s + 1) / NN);
+      const T *xp = x_of(a, bn) + (long long)((s + 1) % NN) * M;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+      {
+        int q = tid + NT * r;
+        if (M % NT == 0 || q < M)
+          xr[r] = gld(xp + q);
+      }
+    }
+    __syncthreads();
+    trailing_stages<T, D, N, NT, EXACT, 2, D - 1>(buf, a, b, tid);
+    T *wb = w_of(a, b, p - e0) + (long long)js * M;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      int q = tid + NT * r;
+      if (M % NT == 0 || q < M)
+        gst(wb + q, buf[q]);
+    }
+  }
+}
+
+// pass 2: blockIdx.y selects NT consecutive positions m; blockIdx.x selects a
+// range of order positions (plan ranges, or chunks of a.chunk from e0).
+template<typename T, int D, int N, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void kron_pair(EntryArgs<T> a, int e0, int e1)
+{
+  using C = typename Acc<T>::type;
+  constexpr int M = cpow(N, D - 2);
+  const int tid = threadIdx.x;
+  const int m = blockIdx.y * NT + tid;
+  int p0, p1;
+  if (a.ranges)
+  {
+    p0 = cld(a.ranges + blockIdx.x);
+    p1 = cld(a.ranges + blockIdx.x + 1);
+  }
+  else
+  {
+    p0 = e0 + blockIdx.x * a.chunk;
+    p1 = min(e1, p0 + a.chunk);
+  }
+  const bool live = (M % NT == 0) || (m < M);
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    const T *wb = w_of(a, b, p - e0);
+    C z[N][N];
+    if (live)
+    {
+#pragma unroll
+      for (int j1 = 0; j1 < N; j1++)
+#pragma unroll
+        for (int j2 = 0; j2 < N; j2++)
+          z[j1][j2] = static_cast<C>(gld(wb + (long long)(j1 * N + j2) * M + m));
+    }
+    constexpr int RB = (N <= 6) ? N : (N + 1) / 2;
+    {
+      // mode 2: u(j1, i2) = sum_j2 A_2(i2, j2) z(j1, j2)
+      const T *A2 = a_of(a, b, 1, D);
+      C u[N][N];
+#pragma unroll
+      for (int r0 = 0; r0 < N; r0 += RB)
+      {
+        C am[RB][N];
+#pragma unroll
+        for (int j = 0; j < N; j++)
+#pragma unroll
+          for (int i = 0; i < RB; i++)
+            if (r0 + i < N)
+              am[i][j] = static_cast<C>(cld(A2 + (r0 + i + j * a.lda)));
+#pragma unroll
+        for (int j1 = 0; j1 < N; j1++)
+#pragma unroll
+          for (int i = 0; i < RB; i++)
+            if (r0 + i < N)
+            {
+              C acc = 0;
+#pragma unroll
+              for (int j2 = 0; j2 < N; j2++)
+                acc = madd<EXACT>(acc, am[i][j2], z[j1][j2]);
+              u[j1][r0 + i] = static_cast<C>(static_cast<T>(acc));
+            }
+      }
+      // mode 1: z(i1, i2) = sum_j1 A_1(i1, j1) u(j1, i2)
+      const T *A1 = a_of(a, b, 0, D);
+#pragma unroll
+      for (int r0 = 0; r0 < N; r0 += RB)
+      {
+        C am[RB][N];
+#pragma unroll
+        for (int j = 0; j < N; j++)
+#pragma unroll
+          for (int i = 0; i < RB; i++)
+            if (r0 + i < N)
+              am[i][j] = static_cast<C>(cld(A1 + (r0 + i + j * a.lda)));
+#pragma unroll
+        for (int i2 = 0; i2 < N; i2++)
+#pragma unroll
+          for (int i = 0; i < RB; i++)
+            if (r0 + i < N)
+            {
+              C acc = 0;
+#pragma unroll
+              for (int j1 = 0; j1 < N; j1++)
+                acc = madd<EXACT>(acc, am[i][j1], u[j1][i2]);
+              z[r0 + i][i2] = acc;
+            }
+      }
+    }
+    if (live)
+    {
+      T *yb = y_of(a, b);
+#pragma unroll
+      for (int i1 = 0; i1 < N; i1++)
+      {
+        if (a.owned)
+        {
+          T yv[N];
+#pragma unroll
+          for (int i2 = 0; i2 < N; i2++)
+            yv[i2] = gld(yb + (long long)(i1 * N + i2) * M + m);
+#pragma unroll
+          for (int i2 = 0; i2 < N; i2++)
+            gst(yb + (long long)(i1 * N + i2) * M + m,
+                yv[i2] + static_cast<T>(z[i1][i2]));
+        }
+        else
+        {
+#pragma unroll
+          for (int i2 = 0; i2 < N; i2++)
+            gatomic_add(yb + (long long)(i1 * N + i2) * M + m,
+                        static_cast<T>(z[i1][i2]));
+        }
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -13,7 +13,8 @@ enum Family
 {
   FAM_LDS = 0,
   FAM_SLAB = 1,
-  FAM_GLOBAL = 2
+  FAM_GLOBAL = 2,
+  FAM_TWOPASS = 3
 };
 
 struct KernelInfo
@@ -38,6 +39,11 @@ bool lds_kernel_d(int n, bool exact, bool fwd, KernelInfo *out);
 // kron_slab instantiation for (d, n) if one was built
 template<typename T>
 bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
+
+// kron_sub + kron_pair for (d, n) if built
+template<typename T>
+bool twopass_kernels(int d, int n, bool exact, KernelInfo *sub,
+                     KernelInfo *pair);
 
 // kron_global (always available)
 template<typename T>

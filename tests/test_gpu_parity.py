@@ -55,7 +55,8 @@ def test_pbatched_overlap_vs_oracle(km, d, n):
     assert rel_err(got, want) <= RTOL_FP64, (d, n, rel_err(got, want))
 
 
-@pytest.mark.parametrize("d,n", [(1, 5), (2, 7), (3, 8), (4, 4), (4, 6), (5, 3), (5, 7), (6, 2), (6, 4), (6, 5), (6, 6), (5, 8)])
+@pytest.mark.parametrize("d,n", [(1, 5), (2, 7), (3, 8), (4, 4), (4, 6), (5, 3), (5, 7), (6, 2), (6, 4), (6, 5), (6, 6),
+                                 (5, 8), (6, 7), (6, 8)])
 @pytest.mark.parametrize("variant", ["backward", "forward"])
 def test_exact_mode_bitwise(km, d, n, variant):
     batch = _batch_for(d, n)
@@ -291,3 +292,56 @@ def test_empty_batch_is_noop(km):
     km.kronmult_pbatched(2, 4, Y, Y, Y, None, 0)
     torch.cuda.synchronize()
     assert torch.equal(Y, torch.ones_like(Y))
+
+
+@pytest.mark.parametrize("d,n", [(6, 7), (6, 8)])
+def test_twopass_family_selected(km, d, n):
+    assert km.query_kernel(d, n, 8)[0] == km.FAMILY_TWOPASS
+    assert km.query_kernel(d, n, 4)[0] == km.FAMILY_TWOPASS
+
+
+def test_twopass_xbatched_chunked_by_subbatch(km):
+    """d=6, n=7 through xbatched with subbatchCount=2: the two passes run in
+    chunks of two entries, each using the workspace pair of its slot."""
+    rng = np.random.default_rng(77)
+    d, n, batch, lda = 6, 7, 5, 8
+    pools = [rng.uniform(-1, 1, (3, n, lda)) for _ in range(d)]
+    aidx = rng.integers(0, 3, (batch, d))
+    X = rng.uniform(-1, 1, (3, n ** d))
+    xidx = np.array([0, 1, 2, 0, 1])
+    Y0 = rng.uniform(-1, 1, (2, n ** d))
+    yidx = np.array([0, 1, 1, 0, 0])
+    want = O.xbatched(pools, aidx, X, xidx, Y0, yidx, variant=O.BACKWARD)
+    got = gpu_xbatched(pools, aidx, X, xidx, Y0, yidx, subbatch=2)
+    assert rel_err(got, want) <= RTOL_FP64
+
+
+def test_twopass_vbatched_square_chunked(km):
+    """Square vbatched at d=6, n=7 with room for two slot pairs only."""
+    rng = np.random.default_rng(78)
+    d, n, batch = 6, 7, 3
+    m = (n,) * d
+    mats = [[rng.uniform(-1, 1, (n, n)) for _ in range(d)] for _ in range(batch)]
+    X = rng.uniform(-1, 1, (batch, n ** d))
+    Y0 = rng.uniform(-1, 1, (2, n ** d))
+    yidx = np.array([0, 1, 0])
+    want = O.vbatched(m, m, mats, X, Y0, yidx, variant=O.BACKWARD)
+    got = gpu_vbatched(m, m, mats, X, Y0, yidx, Wcap=2 * 2 * n ** d * 8)
+    assert rel_err(got, want) <= RTOL_FP64
+
+
+def test_twopass_plan_deterministic(km):
+    import torch
+    d, n, batch = 6, 8, 6
+    A, X, Y0 = rng_inputs(79, d, n, batch, 2)
+    yidx = np.array([0, 0, 1, 1, 0, 1])
+    dev = torch.device("cuda:0")
+    tY = torch.tensor(Y0, device=dev)
+    plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
+    opt = km.options(plan=plan)
+    g1 = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
+    g2 = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
+    assert np.array_equal(g1, g2)
+    want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
+    assert rel_err(g1, want) <= RTOL_FP64
+    plan.close()
