@@ -68,7 +68,7 @@ def test_exact_mode_bitwise(km, d, n, variant):
     else:
         opt = km.options(exact=True, variant=km.VARIANT_FORWARD)
         want = O.pbatched(A, X, Y0, yidx, variant=O.RECURSIVE)
-    with_w = km.query_kernel(d, n, 8, opt)[0] == km.FAMILY_GLOBAL
+    with_w = km.query_kernel(d, n, 8, opt)[0] in (km.FAMILY_GLOBAL, km.FAMILY_TWOPASS)
     got = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=with_w)
     assert np.array_equal(got, want), (d, n, variant, rel_err(got, want))
 
