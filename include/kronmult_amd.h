@@ -171,6 +171,41 @@ km_status km_plan_destroy(km_plan plan);
 km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
 
 /* ---------------------------------------------------------------------------
+ * Grouped variable-size batch: entries with their own square size n_b
+ * (adaptive sparse grids), all with the same d, in ONE persistent launch.
+ * Extends the reference's uniform-shape vbatched (kronmult_vbatched.hpp:
+ * 19-24, where every entry shares m_k, n_k) to per-entry sizes; there is no
+ * counterpart in the reference.
+ *   Aarray: d*batchCount device pointers, A_k of entry b at Aarray[k + d*b],
+ *           n_b x n_b with leading dimension n_b;
+ *   pX/pY:  batchCount device pointers to vectors of n_b^d elements
+ *           (outputs may repeat; entries sharing an output must have the
+ *           same n_b).
+ * The group plan bins the entries by size, groups them by output and splits
+ * them over workgroups balanced by flops; each output is owned by one
+ * workgroup (deterministic, no atomics).  It reads the host array n_host and
+ * the device pointer array pY once; calls must pass the same pY contents.
+ * Supported sizes: n_b <= 8 for d <= 4, n_b <= 6 for d = 5, n_b <= 4 for
+ * d = 6 (KM_ERROR_NOT_SUPPORTED otherwise).
+ * ------------------------------------------------------------------------ */
+typedef struct km_group_plan_s *km_group_plan;
+
+km_status km_group_plan_create(km_group_plan *plan, int d, int batchCount,
+                               const int *n_host, const void *const *pY,
+                               int elem_bytes, void *stream);
+km_status km_group_plan_destroy(km_group_plan plan);
+km_status km_group_plan_info(km_group_plan plan, int *num_outputs,
+                             int *num_groups, int *max_n);
+km_status km_kronmult_grouped_f64(km_group_plan plan,
+                                  const double *const *Aarray,
+                                  double *const *pX, double *const *pY,
+                                  void *stream, const km_options *opt);
+km_status km_kronmult_grouped_f32(km_group_plan plan,
+                                  const float *const *Aarray, float *const *pX,
+                                  float *const *pY, void *stream,
+                                  const km_options *opt);
+
+/* ---------------------------------------------------------------------------
  * Introspection: which kernel family a square (d, n) call would use.
  *   *family: 0 = LDS-resident, 1 = slab-streamed, 2 = workspace-staged
  *            (generic), 3 = two-pass (sub-slabs in LDS, then the two

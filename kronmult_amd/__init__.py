@@ -57,6 +57,8 @@ C_ABI_SYMBOLS = (
     "km_kronmult_vbatched_f64", "km_kronmult_vbatched_f32",
     "km_plan_create", "km_plan_destroy", "km_plan_info",
     "km_query_kernel", "km_kron_flops",
+    "km_group_plan_create", "km_group_plan_destroy", "km_group_plan_info",
+    "km_kronmult_grouped_f64", "km_kronmult_grouped_f32",
 )
 
 
@@ -121,6 +123,16 @@ def lib():
         L.km_query_kernel.restype = _I
         L.km_kron_flops.argtypes = [_I, _P, _P]
         L.km_kron_flops.restype = C.c_double
+        L.km_group_plan_create.argtypes = [C.POINTER(C.c_void_p), _I, _I, _P, _P, _I, _P]
+        L.km_group_plan_create.restype = _I
+        L.km_group_plan_destroy.argtypes = [_P]
+        L.km_group_plan_destroy.restype = _I
+        L.km_group_plan_info.argtypes = [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]
+        L.km_group_plan_info.restype = _I
+        for suf in ("f64", "f32"):
+            f = getattr(L, f"km_kronmult_grouped_{suf}")
+            f.argtypes = [_P, _P, _P, _P, _P, C.POINTER(Options)]
+            f.restype = _I
         _lib = L
     return _lib
 
@@ -256,6 +268,44 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class GroupPlan:
+    """Plan of a grouped variable-size batch: per-entry sizes ``n_host``
+    (host integers) and the device output pointer array ``pY``."""
+
+    def __init__(self, d: int, n_host, pY, elem_bytes: int = 8, stream=None):
+        nn = (C.c_int * len(n_host))(*[int(v) for v in n_host])
+        h = C.c_void_p()
+        self.batch = len(n_host)
+        _check(lib().km_group_plan_create(C.byref(h), d, self.batch, C.cast(nn, C.c_void_p), _addr(pY),
+                                          elem_bytes, _stream(stream)), "km_group_plan_create")
+        self.handle = h.value
+        self.d = d
+
+    def info(self):
+        no, ng, nm = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().km_group_plan_info(self.handle, C.byref(no), C.byref(ng), C.byref(nm)), "km_group_plan_info")
+        return no.value, ng.value, nm.value
+
+    def close(self):
+        if self.handle:
+            lib().km_group_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kronmult_grouped(plan: GroupPlan, Aarray, pX, pY, dtype="f64", stream=None,
+                     opt: Optional[Options] = None) -> None:
+    """Grouped variable-size batch in one launch (see km_kronmult_grouped_f64)."""
+    f = getattr(lib(), f"km_kronmult_grouped_{dtype}")
+    _check(f(plan.handle, _addr(Aarray), _addr(pX), _addr(pY), _stream(stream), _opt_ref(opt)),
+           "kronmult_grouped")
 
 
 def query_kernel(d: int, n: int, elem_bytes: int = 8, opt: Optional[Options] = None):

@@ -7,6 +7,7 @@
 //   km_kronmult_xbatched_*  kronmult6_xbatched.cpp:4-28
 //   km_kronmult_vbatched_*  kronmult6_vbatched.hpp:10-34 (template only there)
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -27,6 +28,17 @@ struct km_plan_s
   int device = 0;
   int ngroups = 0;
   int noutputs = 0;
+  int *order = nullptr;  // device, batch entries
+  int *ranges = nullptr; // device, ngroups + 1 entries
+};
+
+struct km_group_plan_s
+{
+  int d = 0, batch = 0, elem_bytes = 0, nmax = 0;
+  int device = 0;
+  int ngroups = 0;
+  int noutputs = 0;
+  int *nvec = nullptr;   // device, per entry
   int *order = nullptr;  // device, batch entries
   int *ranges = nullptr; // device, ngroups + 1 entries
 };
@@ -499,6 +511,53 @@ km_status vbatched(int d, const int *m, const int *n, const T *const *Aarray,
   return launch(c, r, (hipStream_t)stream);
 }
 
+template<typename T>
+km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
+                  T *const *pY, void *stream, const km_options *opt)
+{
+  if (!plan)
+    return KM_ERROR_INVALID_VALUE;
+  if (plan->elem_bytes != (int)sizeof(T))
+    return KM_ERROR_PLAN_MISMATCH;
+  if (plan->batch == 0)
+    return KM_SUCCESS;
+  if (!Aarray || !pX || !pY)
+    return KM_ERROR_INVALID_VALUE;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev != plan->device)
+    return KM_ERROR_PLAN_MISMATCH;
+  km_options o;
+  km_options_init(&o);
+  if (opt)
+    o = *opt;
+  if (o.precision != KM_PRECISION_FP64 || o.plan)
+    return KM_ERROR_NOT_SUPPORTED;
+  const bool exact = (o.flags & KM_FLAG_EXACT) != 0;
+  KernelInfo k;
+  int nmax = 0;
+  if (!km::grouped_kernel<T>(plan->d, exact, &k, &nmax) || plan->nmax > nmax)
+    return KM_ERROR_NOT_SUPPORTED;
+  EntryArgs<T> a{};
+  a.Aptr = Aarray;
+  a.lda = 0; // A_k of entry b has leading dimension n_b
+  a.pX = pX;
+  a.pY = pY;
+  a.batch = plan->batch;
+  a.ranges = plan->ranges;
+  a.order = plan->order;
+  a.owned = 1;
+  const int *nv = plan->nvec;
+  void *args[] = {&a, &nv};
+  if (hipLaunchKernel(k.fn, dim3(plan->ngroups), dim3(k.nt), args, 0,
+                      (hipStream_t)stream) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return KM_ERROR_HIP;
+  }
+  return KM_SUCCESS;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -739,6 +798,155 @@ extern "C"
     if (threads)
       *threads = k.nt;
     return KM_SUCCESS;
+  }
+
+  km_status km_group_plan_create(km_group_plan *plan, int d, int batchCount,
+                                 const int *n_host, const void *const *pY,
+                                 int elem_bytes, void *stream)
+  {
+    if (!plan || d < 1 || d > 6 || batchCount < 0 || (batchCount > 0 && (!n_host || !pY)) ||
+        (elem_bytes != 4 && elem_bytes != 8))
+      return KM_ERROR_INVALID_VALUE;
+    *plan = nullptr;
+    KernelInfo k;
+    int nmax = 0;
+    bool ok = (elem_bytes == 8) ? km::grouped_kernel<double>(d, false, &k, &nmax)
+                                : km::grouped_kernel<float>(d, false, &k, &nmax);
+    if (!ok)
+      return KM_ERROR_NOT_SUPPORTED;
+    int pn = 0;
+    for (int b = 0; b < batchCount; b++)
+    {
+      if (n_host[b] < 1)
+        return KM_ERROR_INVALID_VALUE;
+      if (n_host[b] > nmax)
+        return KM_ERROR_NOT_SUPPORTED;
+      pn = std::max(pn, n_host[b]);
+    }
+    km_group_plan p = new km_group_plan_s;
+    p->d = d;
+    p->batch = batchCount;
+    p->elem_bytes = elem_bytes;
+    p->nmax = pn;
+    (void)hipGetDevice(&p->device);
+    if (batchCount == 0)
+    {
+      *plan = p;
+      return KM_SUCCESS;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<uintptr_t> ys(batchCount);
+    if (hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+    {
+      delete p;
+      return KM_ERROR_HIP;
+    }
+    // size bins, largest first (long entries are scheduled early), then
+    // grouped by output; entry order kept inside an output (stable sort)
+    std::vector<int> order(batchCount);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      if (n_host[x] != n_host[y])
+        return n_host[x] > n_host[y];
+      return ys[x] < ys[y];
+    });
+    // cost of an entry ~ its flops 2 d n^(d+1), plus a fixed per-entry term
+    auto cost = [&](int b) {
+      return 2.0 * d * std::pow((double)n_host[b], d + 1) + 4096.0;
+    };
+    double total = 0;
+    for (int b = 0; b < batchCount; b++)
+      total += cost(b);
+    const long long G = std::max(
+        1LL, std::min<long long>((long long)device_cus() * occupancy(k.fn, k.nt),
+                                 batchCount));
+    const double target = total / G;
+    std::vector<int> ranges;
+    ranges.push_back(0);
+    double in_group = 0;
+    int noutputs = 0;
+    for (int i = 0; i < batchCount;)
+    {
+      int j = i + 1;
+      while (j < batchCount && ys[order[j]] == ys[order[i]] &&
+             n_host[order[j]] == n_host[order[i]])
+        j++;
+      double c = 0;
+      for (int q = i; q < j; q++)
+        c += cost(order[q]);
+      noutputs++;
+      if (in_group > 0 && in_group + c > target)
+      {
+        ranges.push_back(i);
+        in_group = 0;
+      }
+      in_group += c;
+      i = j;
+    }
+    ranges.push_back(batchCount);
+    p->ngroups = (int)ranges.size() - 1;
+    p->noutputs = noutputs;
+    if (hipMalloc(&p->nvec, sizeof(int) * batchCount) != hipSuccess ||
+        hipMalloc(&p->order, sizeof(int) * batchCount) != hipSuccess ||
+        hipMalloc(&p->ranges, sizeof(int) * ranges.size()) != hipSuccess ||
+        hipMemcpy(p->nvec, n_host, sizeof(int) * batchCount,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+    {
+      km_group_plan_destroy(p);
+      return KM_ERROR_HIP;
+    }
+    *plan = p;
+    return KM_SUCCESS;
+  }
+
+  km_status km_group_plan_destroy(km_group_plan plan)
+  {
+    if (!plan)
+      return KM_SUCCESS;
+    if (plan->nvec)
+      (void)hipFree(plan->nvec);
+    if (plan->order)
+      (void)hipFree(plan->order);
+    if (plan->ranges)
+      (void)hipFree(plan->ranges);
+    delete plan;
+    return KM_SUCCESS;
+  }
+
+  km_status km_group_plan_info(km_group_plan plan, int *num_outputs,
+                               int *num_groups, int *max_n)
+  {
+    if (!plan)
+      return KM_ERROR_INVALID_VALUE;
+    if (num_outputs)
+      *num_outputs = plan->noutputs;
+    if (num_groups)
+      *num_groups = plan->ngroups;
+    if (max_n)
+      *max_n = plan->nmax;
+    return KM_SUCCESS;
+  }
+
+  km_status km_kronmult_grouped_f64(km_group_plan plan,
+                                    const double *const *Aarray,
+                                    double *const *pX, double *const *pY,
+                                    void *stream, const km_options *opt)
+  {
+    return grouped<double>(plan, Aarray, pX, pY, stream, opt);
+  }
+
+  km_status km_kronmult_grouped_f32(km_group_plan plan,
+                                    const float *const *Aarray, float *const *pX,
+                                    float *const *pY, void *stream,
+                                    const km_options *opt)
+  {
+    return grouped<float>(plan, Aarray, pX, pY, stream, opt);
   }
 
   double km_kron_flops(int d, const int *m, const int *n)

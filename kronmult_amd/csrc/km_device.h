@@ -334,7 +334,9 @@ __device__ __forceinline__ void lds_stage(T *buf, const EntryArgs<T> &a, int b,
   constexpr int MODE = FWD ? K : D - 1 - K;
   constexpr int STRIDE = cpow(N, D - 1 - MODE);
   C am[N][N];
-  load_a<N>(am, a_of(a, b, MODE, D), a.lda);
+  // lda 0: each A_k is stored densely (leading dimension n), as in the
+  // grouped entry point where n differs from entry to entry
+  load_a<N>(am, a_of(a, b, MODE, D), a.lda ? a.lda : N);
 #pragma unroll kLdsUnroll
   for (int f = tid; f < F; f += NT)
   {
@@ -578,6 +580,130 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
 #pragma unroll
   for (int i1 = 0; i1 < N; i1++)
     flush_run<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned);
+}
+
+// ---------------------------------------------------------------------------
+// kron_grouped: one persistent launch over entries of different sizes (square
+// n_b x n_b factors, n_b <= NMAX, all with the same d).  Entries arrive sorted
+// by size and output (a group plan), every workgroup walks its range and each
+// entry dispatches to the compile-time body of its n: the LDS-resident
+// algorithm of kron_lds with run merging of consecutive equal outputs.
+// ---------------------------------------------------------------------------
+template<int R, int NT, typename T>
+__device__ __forceinline__ void flush_dyn(T *dst, const T (&acc)[R], int tid,
+                                          int owned, int size)
+{
+  if (owned)
+  {
+    T yv[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (q < size)
+        yv[r] = gld(dst + q);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (q < size)
+        gst(dst + q, yv[r] + acc[r]);
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (q < size)
+        gatomic_add(dst + q, acc[r]);
+    }
+  }
+}
+
+template<typename T, int D, int N, int NT, int R, bool EXACT>
+__device__ __forceinline__ void grouped_entry(T *buf, const EntryArgs<T> &a,
+                                              int b, int tid, T (&acc)[R])
+{
+  constexpr int S = cpow(N, D);
+  static_assert(S <= R * NT, "accumulator too small for this size");
+  const T *xp = x_of(a, b);
+#pragma unroll
+  for (int r = 0; r < (S + NT - 1) / NT; r++)
+  {
+    const int q = tid + NT * r;
+    if (S % NT == 0 || q < S)
+      buf[q] = gld(xp + q);
+  }
+  __syncthreads();
+  lds_stage<T, D, N, NT, EXACT, false, 0>(buf, a, b, tid);
+#pragma unroll
+  for (int r = 0; r < (S + NT - 1) / NT; r++)
+  {
+    const int q = tid + NT * r;
+    if (S % NT == 0 || q < S)
+      acc[r] = acc[r] + buf[q];
+  }
+  __syncthreads();
+}
+
+template<typename T, int D, int NMAX, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void kron_grouped(EntryArgs<T> a,
+                                                   const int *nvec)
+{
+  constexpr int SMAX = cpow(NMAX, D);
+  constexpr int R = (SMAX + NT - 1) / NT;
+  __shared__ T buf[SMAX];
+  const int tid = threadIdx.x;
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+  T acc[R];
+  T *target = nullptr;
+  int tsize = 0;
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    const int n = cld(nvec + b);
+    int size = 1;
+    for (int k = 0; k < D; k++)
+      size *= n;
+    T *yb = y_of(a, b);
+    if (yb != target)
+    {
+      if (target)
+        flush_dyn<R, NT>(target, acc, tid, a.owned, tsize);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        acc[r] = 0;
+      target = yb;
+      tsize = size;
+    }
+    switch (n)
+    {
+#define KM_GROUPED_N(NN)                                                       \
+  case NN:                                                                     \
+    if constexpr (NN <= NMAX)                                                  \
+      grouped_entry<T, D, NN, NT, R, EXACT>(buf, a, b, tid, acc);              \
+    break;
+      KM_GROUPED_N(1)
+      KM_GROUPED_N(2)
+      KM_GROUPED_N(3)
+      KM_GROUPED_N(4)
+      KM_GROUPED_N(5)
+      KM_GROUPED_N(6)
+      KM_GROUPED_N(7)
+      KM_GROUPED_N(8)
+#undef KM_GROUPED_N
+    default:
+      break;
+    }
+  }
+  if (target)
+    flush_dyn<R, NT>(target, acc, tid, a.owned, tsize);
 }
 
 // ---------------------------------------------------------------------------

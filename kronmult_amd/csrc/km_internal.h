@@ -14,7 +14,8 @@ enum Family
   FAM_LDS = 0,
   FAM_SLAB = 1,
   FAM_GLOBAL = 2,
-  FAM_TWOPASS = 3
+  FAM_TWOPASS = 3,
+  FAM_GROUPED = 4
 };
 
 struct KernelInfo
@@ -44,6 +45,10 @@ bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
 template<typename T>
 bool twopass_kernels(int d, int n, bool exact, KernelInfo *sub,
                      KernelInfo *pair);
+
+// kron_grouped for d (per-entry n up to *nmax)
+template<typename T>
+bool grouped_kernel(int d, bool exact, KernelInfo *out, int *nmax);
 
 // kron_global (always available)
 template<typename T>

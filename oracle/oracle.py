@@ -291,7 +291,9 @@ def entry(mats: Sequence[np.ndarray], x: np.ndarray, variant: int = RECURSIVE) -
     ld = _ivec([a.shape[0] for a in mats])
     Aptr = np.array([_ptr(c) for c in cols], dtype=np.uintp)
     xs = int(np.prod(n))
-    ws = max([xs, int(np.prod(m))] + [int(np.prod(m[:L])) * int(np.prod(n[L:])) for L in range(1, d)])
+    # stage sizes of both orders: forward (modes 1..d) and backward (d..1)
+    ws = max([xs, int(np.prod(m))] + [int(np.prod(m[:L])) * int(np.prod(n[L:])) for L in range(1, d)]
+             + [int(np.prod(n[:d - L])) * int(np.prod(m[d - L:])) for L in range(1, d)])
     xx = np.zeros(ws, dtype=dt)
     xx[:xs] = x
     w = np.zeros(ws, dtype=dt)

@@ -1,0 +1,103 @@
+"""CPU tests of the C ABI boundary: the shared library loads without a GPU,
+exports every function include/kronmult_amd.h declares (and the
+reference-named C++ overloads of include/kronmult_compat.hpp), and rejects
+invalid arguments with the documented status before touching any device."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "kronmult_amd", "libkronmult_amd.so")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "kronmult_amd.h")).read()
+    return sorted(set(re.findall(r"\b(km_[a-z0-9_]+)\s*\(", text)))
+
+
+def _exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_every_declared_function_is_exported():
+    names = _header_functions()
+    assert len(names) >= 20
+    exported = _exported()
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+
+
+def test_python_mirror_lists_the_abi():
+    import kronmult_amd as km
+    assert set(km.C_ABI_SYMBOLS) == set(_header_functions())
+
+
+def test_compat_overloads_exported():
+    out = subprocess.run(["nm", "-DC", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    for kind in ("batched", "pbatched", "xbatched", "vbatched"):
+        for n in range(1, 7):
+            assert re.search(rf"\bkronmult{n}_{kind}\(", out), f"kronmult{n}_{kind}"
+    hdr = open(os.path.join(ROOT, "include", "kronmult_compat.hpp")).read()
+    assert hdr.count("void kronmult") == 24
+
+
+def test_invalid_arguments_rejected_on_host():
+    import kronmult_amd as km
+    L = km.lib()
+    f = L.km_kronmult_pbatched_f64
+    assert f(0, 4, None, None, None, None, 1, None, None) == km.ERROR_INVALID_VALUE   # d < 1
+    assert f(7, 4, None, None, None, None, 1, None, None) == km.ERROR_INVALID_VALUE   # d > 6
+    assert f(3, 0, None, None, None, None, 1, None, None) == km.ERROR_INVALID_VALUE   # n < 1
+    assert f(3, 4, None, None, None, None, -1, None, None) == km.ERROR_INVALID_VALUE  # batch < 0
+    assert f(3, 4, None, None, None, None, 5, None, None) == km.ERROR_INVALID_VALUE   # null arrays
+    assert f(6, 40, 1, 1, 1, None, 1, None, None) == km.ERROR_INVALID_VALUE           # n^d > 2^31-1
+    assert f(3, 4, None, None, None, None, 0, None, None) == km.SUCCESS               # empty batch
+    x = L.km_kronmult_xbatched_f64
+    assert x(3, 4, 1, 3, 1, 1, None, 4, 0, None, None) == km.ERROR_INVALID_VALUE      # lda < n
+    assert x(3, 4, 1, 4, 1, 1, None, 4, -1, None, None) == km.ERROR_INVALID_VALUE     # subbatch < 0
+    v = L.km_kronmult_vbatched_f64
+    m = (C.c_int * 2)(2, 0)
+    assert v(2, C.cast(m, C.c_void_p), C.cast(m, C.c_void_p), 1, 1, 1, None, 0, 1, None, None) == \
+        km.ERROR_INVALID_VALUE
+    o = km.Options()
+    L.km_options_init(C.byref(o))
+    o.precision = 7
+    assert f(3, 4, 1, 1, 1, None, 2, None, C.byref(o)) == km.ERROR_NOT_SUPPORTED
+    o.precision = 0
+    o.variant = 9
+    assert f(3, 4, 1, 1, 1, None, 2, None, C.byref(o)) == km.ERROR_INVALID_VALUE
+
+
+def test_options_defaults_and_status_strings():
+    import kronmult_amd as km
+    o = km.Options()
+    km.lib().km_options_init(C.byref(o))
+    assert (o.flags, o.variant, o.precision, o.plan) == (0, km.VARIANT_BACKWARD, km.PRECISION_FP64, None)
+    for s in range(6):
+        assert km.status_string(s) and "unknown" not in km.status_string(s)
+
+
+def test_kron_flops_matches_reference_model():
+    import kronmult_amd as km
+    from oracle import oracle as O
+    for m, n in (((6,) * 6, (6,) * 6), ((2, 3, 4), (4, 3, 2)), ((8,) * 3, (8,) * 3), ((5,), (7,))):
+        assert km.kron_flops(m, n) == O.kron_flops(m, n)
+
+
+def test_query_kernel_families():
+    import kronmult_amd as km
+    assert km.query_kernel(6, 4)[0] == km.FAMILY_LDS
+    assert km.query_kernel(3, 8)[0] == km.FAMILY_LDS
+    assert km.query_kernel(6, 6)[0] == km.FAMILY_SLAB
+    assert km.query_kernel(5, 8)[0] == km.FAMILY_SLAB
+    assert km.query_kernel(6, 8)[0] == km.FAMILY_TWOPASS
+    assert km.query_kernel(3, 12)[0] == km.FAMILY_GLOBAL
+    assert km.query_kernel(6, 4, opt=km.options(force_global=True))[0] == km.FAMILY_GLOBAL

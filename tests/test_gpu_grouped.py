@@ -1,0 +1,103 @@
+"""GPU parity of the grouped variable-size entry point (per-entry n, one
+persistent launch) against the CPU oracle, entry by entry."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _helpers import RTOL_FP64, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _make(seed, d, sizes, batch, ngroups_per_n):
+    """Entries with n drawn from `sizes`; outputs form groups of equal n and
+    every entry targets a random group of its own n."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(sizes, batch)
+    groups = {n: list(range(ngroups_per_n)) for n in sizes}
+    yg = np.array([rng.integers(0, ngroups_per_n) for _ in range(batch)])
+    mats = [[rng.uniform(-1, 1, (n, n)) for _ in range(d)] for n in ns]
+    X = [rng.uniform(-1, 1, n ** d) for n in ns]
+    Y0 = {n: rng.uniform(-1, 1, (ngroups_per_n, n ** d)) for n in sizes}
+    del groups
+    return ns, yg, mats, X, Y0
+
+
+def _oracle(d, ns, yg, mats, X, Y0):
+    Y = {n: v.copy() for n, v in Y0.items()}
+    for b, n in enumerate(ns):
+        Y[n][yg[b]] += O.entry(mats[b], X[b], O.BACKWARD)
+    return Y
+
+
+def _gpu(km, d, ns, yg, mats, X, Y0, exact=False, runs=1):
+    import torch
+    dev = torch.device("cuda:0")
+    batch = len(ns)
+    tY = {n: torch.tensor(v, device=dev) for n, v in Y0.items()}
+    tX = [torch.tensor(x, device=dev) for x in X]
+    # A_k of entry b: column-major n x n (mats hold A as (row, col))
+    tA = [torch.tensor(np.ascontiguousarray(mats[b][k].T), device=dev) for b in range(batch) for k in range(d)]
+    table = torch.tensor([t.data_ptr() for t in tA], dtype=torch.int64, device=dev)
+    pX = torch.tensor([t.data_ptr() for t in tX], dtype=torch.int64, device=dev)
+    pY = torch.tensor([tY[n][yg[b]].data_ptr() for b, n in enumerate(ns)], dtype=torch.int64, device=dev)
+    plan = km.GroupPlan(d, list(ns), pY)
+    outs = []
+    for _ in range(runs):
+        for n in tY:
+            tY[n].copy_(torch.tensor(Y0[n], device=dev))
+        km.kronmult_grouped(plan, table, pX, pY, opt=km.options(exact=exact))
+        torch.cuda.synchronize()
+        outs.append({n: t.cpu().numpy() for n, t in tY.items()})
+    info = plan.info()
+    plan.close()
+    return outs, info
+
+
+@pytest.fixture(scope="module")
+def km():
+    import kronmult_amd
+    kronmult_amd.lib()
+    return kronmult_amd
+
+
+@pytest.mark.parametrize("d,sizes", [(4, [2, 3, 4, 5, 6, 7, 8]), (3, [1, 2, 5, 8]), (2, [3, 8]),
+                                     (5, [2, 3, 6]), (6, [2, 3, 4]), (1, [4, 7])])
+def test_grouped_vs_oracle(km, d, sizes):
+    ns, yg, mats, X, Y0 = _make(d * 10 + len(sizes), d, sizes, 300, 5)
+    want = _oracle(d, ns, yg, mats, X, Y0)
+    outs, (nout, ngroups, nmax) = _gpu(km, d, ns, yg, mats, X, Y0, runs=2)
+    assert nmax == max(ns)
+    assert nout == len({(n, g) for n, g in zip(ns, yg)})
+    for n in want:
+        assert rel_err(outs[0][n], want[n]) <= RTOL_FP64, n
+        # deterministic: owned outputs, fixed order
+        assert np.array_equal(outs[0][n], outs[1][n])
+
+
+def test_grouped_exact_single_entry_outputs_bitwise(km):
+    d = 4
+    rng = np.random.default_rng(4)
+    ns = np.array([2, 8, 5, 3, 7, 6, 4, 8])
+    yg = np.arange(len(ns))
+    mats = [[rng.uniform(-1, 1, (n, n)) for _ in range(d)] for n in ns]
+    X = [rng.uniform(-1, 1, n ** d) for n in ns]
+    Y0 = {}
+    for b, n in enumerate(ns):
+        Y0.setdefault(n, np.zeros((len(ns), n ** d)))
+    want = _oracle(d, ns, yg, mats, X, Y0)
+    outs, _ = _gpu(km, d, ns, yg, mats, X, Y0, exact=True)
+    for n in want:
+        assert np.array_equal(outs[0][n], want[n])
+
+
+def test_grouped_rejects_oversize(km):
+    import torch
+    dev = torch.device("cuda:0")
+    Y = torch.zeros(4096 * 2, dtype=torch.float64, device=dev)
+    pY = torch.tensor([Y.data_ptr()] * 2, dtype=torch.int64, device=dev)
+    with pytest.raises(km.KronmultError) as ei:
+        km.GroupPlan(6, [4, 5], pY)
+    assert ei.value.status == km.ERROR_NOT_SUPPORTED
