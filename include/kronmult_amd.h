@@ -85,6 +85,10 @@ void km_options_init(km_options *opt);
 
 const char *km_status_string(km_status s);
 
+/* Detail of the last KM_ERROR_HIP on the calling thread (failing HIP call and
+ * its error string); "" when none occurred. */
+const char *km_last_error(void);
+
 /* ---------------------------------------------------------------------------
  * kronmultN_batched  (kronmult6_batched.cpp:4-18, kronmult6_batched.hpp:12-75)
  *   A: (n, n, d, batchCount) contiguous; X, Y, W: (n^d, batchCount)

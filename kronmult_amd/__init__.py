@@ -50,7 +50,7 @@ FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL, FAMILY_TWOPASS = 0, 1, 2, 3
 
 # every symbol the C ABI exports (include/kronmult_amd.h)
 C_ABI_SYMBOLS = (
-    "km_options_init", "km_status_string",
+    "km_options_init", "km_status_string", "km_last_error",
     "km_kronmult_batched_f64", "km_kronmult_batched_f32",
     "km_kronmult_pbatched_f64", "km_kronmult_pbatched_f32",
     "km_kronmult_xbatched_f64", "km_kronmult_xbatched_f32",
@@ -65,7 +65,13 @@ C_ABI_SYMBOLS = (
 class KronmultError(RuntimeError):
     def __init__(self, status: int, where: str):
         self.status = status
-        super().__init__(f"{where}: {status_string(status)} (status {status})")
+        detail = ""
+        if status == ERROR_HIP:
+            try:
+                detail = " [" + lib().km_last_error().decode() + "]"
+            except Exception:
+                pass
+        super().__init__(f"{where}: {status_string(status)} (status {status}){detail}")
 
 
 class Options(C.Structure):
@@ -95,11 +101,22 @@ def lib():
             raise RuntimeError(
                 f"kronmult_amd: native library not built ({LIB_PATH}); run "
                 "`python -c 'import __graft_entry__ as g; g.build()'`")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7
+        # and names it differently in its DT_NEEDED entries, so if this
+        # library were loaded first (binding the system runtime) torch would
+        # bring up a second runtime, and device pointers and streams would not
+        # be shared.  Loading torch first makes this library bind to torch's.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.km_options_init.argtypes = [C.POINTER(Options)]
         L.km_options_init.restype = None
         L.km_status_string.argtypes = [_I]
         L.km_status_string.restype = C.c_char_p
+        L.km_last_error.argtypes = []
+        L.km_last_error.restype = C.c_char_p
         for suf in ("f64", "f32"):
             f = getattr(L, f"km_kronmult_batched_{suf}")
             f.argtypes = [_I, _I, _P, _P, _P, _P, _I, _P, C.POINTER(Options)]

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -47,6 +48,28 @@ namespace
 {
 
 std::mutex g_mu;
+
+// detail of the last failure on this thread (km_last_error)
+thread_local char g_last_error[256] = "";
+
+void set_error(const char *where, hipError_t e)
+{
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%d)", where,
+                hipGetErrorString(e), (int)e);
+}
+
+// evaluate a HIP call; on failure record where and return KM_ERROR_HIP
+#define KM_HIP(call)                                                           \
+  do                                                                           \
+  {                                                                            \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+    {                                                                          \
+      set_error(#call, e_);                                                    \
+      (void)hipGetLastError();                                                 \
+      return KM_ERROR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
 
 int device_cus()
 {
@@ -221,12 +244,7 @@ km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
     c.a.ranges = nullptr;
     int grid1 = (nsub + c.a.chunk - 1) / c.a.chunk;
     void *args1[] = {&c.a, &e0i, &nsub};
-    if (hipLaunchKernel(k1.fn, dim3(grid1), dim3(k1.nt), args1, 0, stream) !=
-        hipSuccess)
-    {
-      (void)hipGetLastError();
-      return KM_ERROR_HIP;
-    }
+    KM_HIP(hipLaunchKernel(k1.fn, dim3(grid1), dim3(k1.nt), args1, 0, stream));
     // pass 2
     int gx;
     if (r.plan)
@@ -244,12 +262,7 @@ km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
       gx = (cnt + c.a.chunk - 1) / c.a.chunk;
     }
     void *args2[] = {&c.a, &e0i, &e1i};
-    if (hipLaunchKernel(k2.fn, dim3(gx, gy), dim3(k2.nt), args2, 0, stream) !=
-        hipSuccess)
-    {
-      (void)hipGetLastError();
-      return KM_ERROR_HIP;
-    }
+    KM_HIP(hipLaunchKernel(k2.fn, dim3(gx, gy), dim3(k2.nt), args2, 0, stream));
   }
   return KM_SUCCESS;
 }
@@ -283,7 +296,10 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
   }
   int cus = device_cus();
   if (cus <= 0)
+  {
+    std::snprintf(g_last_error, sizeof(g_last_error), "no device / CU count");
     return KM_ERROR_HIP;
+  }
   int grid = 0;
   if (r.plan)
   {
@@ -329,6 +345,7 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
   }
   if (e != hipSuccess)
   {
+    set_error("hipLaunchKernel", e);
     (void)hipGetLastError();
     return KM_ERROR_HIP;
   }
@@ -549,12 +566,8 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
   a.owned = 1;
   const int *nv = plan->nvec;
   void *args[] = {&a, &nv};
-  if (hipLaunchKernel(k.fn, dim3(plan->ngroups), dim3(k.nt), args, 0,
-                      (hipStream_t)stream) != hipSuccess)
-  {
-    (void)hipGetLastError();
-    return KM_ERROR_HIP;
-  }
+  KM_HIP(hipLaunchKernel(k.fn, dim3(plan->ngroups), dim3(k.nt), args, 0,
+                         (hipStream_t)stream));
   return KM_SUCCESS;
 }
 
@@ -576,6 +589,8 @@ extern "C"
     opt->precision = KM_PRECISION_FP64;
     opt->plan = nullptr;
   }
+
+  const char *km_last_error(void) { return g_last_error; }
 
   const char *km_status_string(km_status s)
   {
@@ -675,10 +690,9 @@ extern "C"
       return s;
     hipStream_t st = (hipStream_t)stream;
     std::vector<uintptr_t> ys(batchCount);
-    if (hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
-                       hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return KM_ERROR_HIP;
+    KM_HIP(hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
+                          hipMemcpyDeviceToHost, st));
+    KM_HIP(hipStreamSynchronize(st));
     std::vector<int> order(batchCount);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
@@ -733,15 +747,19 @@ extern "C"
     (void)hipGetDevice(&p->device);
     p->ngroups = (int)ranges.size() - 1;
     p->noutputs = noutputs;
-    if (hipMalloc(&p->order, sizeof(int) * batchCount) != hipSuccess ||
-        hipMalloc(&p->ranges, sizeof(int) * ranges.size()) != hipSuccess ||
-        hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
-                  hipMemcpyHostToDevice) != hipSuccess)
+    km_status st2 = [&]() -> km_status {
+      KM_HIP(hipMalloc(&p->order, sizeof(int) * batchCount));
+      KM_HIP(hipMalloc(&p->ranges, sizeof(int) * ranges.size()));
+      KM_HIP(hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
+                       hipMemcpyHostToDevice));
+      KM_HIP(hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
+                       hipMemcpyHostToDevice));
+      return KM_SUCCESS;
+    }();
+    if (st2 != KM_SUCCESS)
     {
       km_plan_destroy(p);
-      return KM_ERROR_HIP;
+      return st2;
     }
     *plan = p;
     return KM_SUCCESS;
@@ -836,12 +854,18 @@ extern "C"
     }
     hipStream_t st = (hipStream_t)stream;
     std::vector<uintptr_t> ys(batchCount);
-    if (hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
-                       hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
     {
-      delete p;
-      return KM_ERROR_HIP;
+      km_status cs = [&]() -> km_status {
+        KM_HIP(hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
+                              hipMemcpyDeviceToHost, st));
+        KM_HIP(hipStreamSynchronize(st));
+        return KM_SUCCESS;
+      }();
+      if (cs != KM_SUCCESS)
+      {
+        delete p;
+        return cs;
+      }
     }
     // size bins, largest first (long entries are scheduled early), then
     // grouped by output; entry order kept inside an output (stable sort)
@@ -888,18 +912,22 @@ extern "C"
     ranges.push_back(batchCount);
     p->ngroups = (int)ranges.size() - 1;
     p->noutputs = noutputs;
-    if (hipMalloc(&p->nvec, sizeof(int) * batchCount) != hipSuccess ||
-        hipMalloc(&p->order, sizeof(int) * batchCount) != hipSuccess ||
-        hipMalloc(&p->ranges, sizeof(int) * ranges.size()) != hipSuccess ||
-        hipMemcpy(p->nvec, n_host, sizeof(int) * batchCount,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
-                  hipMemcpyHostToDevice) != hipSuccess)
+    km_status st2 = [&]() -> km_status {
+      KM_HIP(hipMalloc(&p->nvec, sizeof(int) * batchCount));
+      KM_HIP(hipMalloc(&p->order, sizeof(int) * batchCount));
+      KM_HIP(hipMalloc(&p->ranges, sizeof(int) * ranges.size()));
+      KM_HIP(hipMemcpy(p->nvec, n_host, sizeof(int) * batchCount,
+                       hipMemcpyHostToDevice));
+      KM_HIP(hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
+                       hipMemcpyHostToDevice));
+      KM_HIP(hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
+                       hipMemcpyHostToDevice));
+      return KM_SUCCESS;
+    }();
+    if (st2 != KM_SUCCESS)
     {
       km_group_plan_destroy(p);
-      return KM_ERROR_HIP;
+      return st2;
     }
     *plan = p;
     return KM_SUCCESS;
