@@ -876,9 +876,15 @@ extern "C"
         return n_host[x] > n_host[y];
       return ys[x] < ys[y];
     });
-    // cost of an entry ~ its flops 2 d n^(d+1), plus a fixed per-entry term
+    // cost of an entry ~ its flops 2 d n^(d+1) plus a fixed per-entry term;
+    // entries small enough to run one per wavefront share the workgroup
+    // (kron_grouped phase 2), so they count 1/waves of their cost
+    const int waves = std::max(1, k.nt / 64);
+    const double small = std::pow((double)nmax, d) / waves;
     auto cost = [&](int b) {
-      return 2.0 * d * std::pow((double)n_host[b], d + 1) + 4096.0;
+      const double sz = std::pow((double)n_host[b], d);
+      const double c = 2.0 * d * sz * n_host[b] + 4096.0;
+      return (waves > 1 && sz <= small) ? c / waves : c;
     };
     double total = 0;
     for (int b = 0; b < batchCount; b++)

@@ -324,7 +324,26 @@ __device__ __forceinline__ void keep_alive(T v)
 //   FWD = true : modes contracted 1..d (the recursive / forward order,
 //                kronmultv_recur.hpp:90-145, kronm_forward.hpp:110-164)
 // ---------------------------------------------------------------------------
-template<typename T, int D, int N, int NT, bool EXACT, bool FWD, int K>
+// Barrier between mode passes: the whole workgroup, or (WAVE) only the
+// wavefront that owns the buffer.  LDS traffic of one wavefront is performed
+// in order; the workgroup-scope fence waits for it and keeps the compiler
+// from moving LDS accesses across the barrier.
+template<bool WAVE>
+__device__ __forceinline__ void pass_sync()
+{
+  if constexpr (WAVE)
+  {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+  else
+  {
+    __syncthreads();
+  }
+}
+
+template<typename T, int D, int N, int NT, bool EXACT, bool FWD, int K,
+         bool WAVE = false>
 __device__ __forceinline__ void lds_stage(T *buf, const EntryArgs<T> &a, int b,
                                           int tid)
 {
@@ -344,9 +363,9 @@ __device__ __forceinline__ void lds_stage(T *buf, const EntryArgs<T> &a, int b,
     int lo = f - hi * STRIDE;
     fiber_apply<N, EXACT, T>(buf + hi * STRIDE * N + lo, STRIDE, am);
   }
-  __syncthreads();
+  pass_sync<WAVE>();
   if constexpr (K + 1 < D)
-    lds_stage<T, D, N, NT, EXACT, FWD, K + 1>(buf, a, b, tid);
+    lds_stage<T, D, N, NT, EXACT, FWD, K + 1, WAVE>(buf, a, b, tid);
 }
 
 template<typename T, int D, int N, int NT, bool EXACT, bool FWD>
@@ -623,7 +642,7 @@ __device__ __forceinline__ void flush_dyn(T *dst, const T (&acc)[R], int tid,
   }
 }
 
-template<typename T, int D, int N, int NT, int R, bool EXACT>
+template<typename T, int D, int N, int NT, int R, bool EXACT, bool WAVE>
 __device__ __forceinline__ void grouped_entry(T *buf, const EntryArgs<T> &a,
                                               int b, int tid, T (&acc)[R])
 {
@@ -637,8 +656,8 @@ __device__ __forceinline__ void grouped_entry(T *buf, const EntryArgs<T> &a,
     if (S % NT == 0 || q < S)
       buf[q] = gld(xp + q);
   }
-  __syncthreads();
-  lds_stage<T, D, N, NT, EXACT, false, 0>(buf, a, b, tid);
+  pass_sync<WAVE>();
+  lds_stage<T, D, N, NT, EXACT, false, 0, WAVE>(buf, a, b, tid);
 #pragma unroll
   for (int r = 0; r < (S + NT - 1) / NT; r++)
   {
@@ -646,15 +665,51 @@ __device__ __forceinline__ void grouped_entry(T *buf, const EntryArgs<T> &a,
     if (S % NT == 0 || q < S)
       acc[r] = acc[r] + buf[q];
   }
-  __syncthreads();
+  pass_sync<WAVE>();
 }
 
+// dispatch one entry of size n to its compile-time body
+template<typename T, int D, int NMAX, int NT, int R, bool EXACT, bool WAVE>
+__device__ __forceinline__ void grouped_dispatch(int n, T *buf,
+                                                 const EntryArgs<T> &a, int b,
+                                                 int tid, T (&acc)[R])
+{
+  switch (n)
+  {
+#define KM_GROUPED_N(NN)                                                       \
+  case NN:                                                                     \
+    if constexpr (NN <= NMAX && cpow(NN, D) <= R * NT)                         \
+      grouped_entry<T, D, NN, NT, R, EXACT, WAVE>(buf, a, b, tid, acc);        \
+    break;
+    KM_GROUPED_N(1)
+    KM_GROUPED_N(2)
+    KM_GROUPED_N(3)
+    KM_GROUPED_N(4)
+    KM_GROUPED_N(5)
+    KM_GROUPED_N(6)
+    KM_GROUPED_N(7)
+    KM_GROUPED_N(8)
+#undef KM_GROUPED_N
+  default:
+    break;
+  }
+}
+
+// Entries of a workgroup's range come largest first (group plan order).
+// Phase 1: entries with n^d > SMALL use the whole workgroup.  Phase 2: the
+// remaining small entries are dealt out to the wavefronts output run by
+// output run (run j goes to wave j mod waves), each wavefront with its own
+// LDS region of SMALL elements and wave-level barriers, so a 16-element
+// entry does not idle 240 threads.  Runs stay whole, so outputs stay owned.
 template<typename T, int D, int NMAX, int NT, bool EXACT>
 __global__ __launch_bounds__(NT) void kron_grouped(EntryArgs<T> a,
                                                    const int *nvec)
 {
   constexpr int SMAX = cpow(NMAX, D);
+  constexpr int NW = NT / 64;          // wavefronts per workgroup
+  constexpr int SMALL = SMAX / NW;     // elements per wavefront region
   constexpr int R = (SMAX + NT - 1) / NT;
+  static_assert((SMALL + 63) / 64 <= R, "per-wave accumulator exceeds R");
   __shared__ T buf[SMAX];
   const int tid = threadIdx.x;
   int p0, p1;
@@ -664,13 +719,17 @@ __global__ __launch_bounds__(NT) void kron_grouped(EntryArgs<T> a,
   T acc[R];
   T *target = nullptr;
   int tsize = 0;
-  for (int p = p0; p < p1; ++p)
+  int p = p0;
+  // phase 1: large entries, whole workgroup
+  for (; p < p1; ++p)
   {
     const int b = entry_at(a, p);
     const int n = cld(nvec + b);
     int size = 1;
     for (int k = 0; k < D; k++)
       size *= n;
+    if (NW > 1 && size <= SMALL)
+      break;
     T *yb = y_of(a, b);
     if (yb != target)
     {
@@ -682,28 +741,50 @@ __global__ __launch_bounds__(NT) void kron_grouped(EntryArgs<T> a,
       target = yb;
       tsize = size;
     }
-    switch (n)
-    {
-#define KM_GROUPED_N(NN)                                                       \
-  case NN:                                                                     \
-    if constexpr (NN <= NMAX)                                                  \
-      grouped_entry<T, D, NN, NT, R, EXACT>(buf, a, b, tid, acc);              \
-    break;
-      KM_GROUPED_N(1)
-      KM_GROUPED_N(2)
-      KM_GROUPED_N(3)
-      KM_GROUPED_N(4)
-      KM_GROUPED_N(5)
-      KM_GROUPED_N(6)
-      KM_GROUPED_N(7)
-      KM_GROUPED_N(8)
-#undef KM_GROUPED_N
-    default:
-      break;
-    }
+    grouped_dispatch<T, D, NMAX, NT, R, EXACT, false>(n, buf, a, b, tid, acc);
   }
   if (target)
     flush_dyn<R, NT>(target, acc, tid, a.owned, tsize);
+  if constexpr (NW > 1)
+  {
+    // phase 2: small entries, one output run per wavefront at a time
+    const int wave = tid >> 6, lane = tid & 63;
+    T *wbuf = buf + wave * SMALL;
+    T acc2[R];
+    T *t2 = nullptr;
+    int tsize2 = 0;
+    T *prev = nullptr;
+    int run = -1;
+    for (; p < p1; ++p)
+    {
+      const int b = entry_at(a, p);
+      T *yb = y_of(a, b);
+      if (yb != prev)
+      {
+        ++run;
+        prev = yb;
+      }
+      if (run % NW != wave)
+        continue;
+      const int n = cld(nvec + b);
+      int size = 1;
+      for (int k = 0; k < D; k++)
+        size *= n;
+      if (yb != t2)
+      {
+        if (t2)
+          flush_dyn<R, 64>(t2, acc2, lane, a.owned, tsize2);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          acc2[r] = 0;
+        t2 = yb;
+        tsize2 = size;
+      }
+      grouped_dispatch<T, D, NMAX, 64, R, EXACT, true>(n, wbuf, a, b, lane, acc2);
+    }
+    if (t2)
+      flush_dyn<R, 64>(t2, acc2, lane, a.owned, tsize2);
+  }
 }
 
 // ---------------------------------------------------------------------------
