@@ -68,6 +68,8 @@ typedef enum km_precision
                                      dot product (the reference's CPU rounding);
                                      default uses fused multiply-add          */
 #define KM_FLAG_FORCE_GLOBAL 0x2u /* always use the workspace-staged kernel   */
+#define KM_FLAG_PERSISTENT 0x4u   /* grouped calls: one persistent launch over
+                                     all sizes instead of one launch per size */
 
 typedef struct km_plan_s *km_plan;
 
@@ -186,8 +188,10 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
  *           (outputs may repeat; entries sharing an output must have the
  *           same n_b).
  * The group plan bins the entries by size, groups them by output and splits
- * them over workgroups balanced by flops; each output is owned by one
- * workgroup (deterministic, no atomics).  It reads the host array n_host and
+ * them over workgroups; each output is owned by one workgroup
+ * (deterministic, no atomics).  By default a call issues one LDS-resident
+ * launch per size bin (back to back on `stream`, capturable in a HIP graph);
+ * KM_FLAG_PERSISTENT runs all bins in a single persistent launch instead.  It reads the host array n_host and
  * the device pointer array pY once; calls must pass the same pY contents.
  * Supported sizes: n_b <= 8 for d <= 4, n_b <= 6 for d = 5, n_b <= 4 for
  * d = 6 (KM_ERROR_NOT_SUPPORTED otherwise).

@@ -33,6 +33,13 @@ struct km_plan_s
   int *ranges = nullptr; // device, ngroups + 1 entries
 };
 
+struct km_group_bin
+{
+  int n;       // entry size of the bin
+  int first;   // first range of the bin in binranges
+  int count;   // ranges of the bin
+};
+
 struct km_group_plan_s
 {
   int d = 0, batch = 0, elem_bytes = 0, nmax = 0;
@@ -41,7 +48,10 @@ struct km_group_plan_s
   int noutputs = 0;
   int *nvec = nullptr;   // device, per entry
   int *order = nullptr;  // device, batch entries
-  int *ranges = nullptr; // device, ngroups + 1 entries
+  int *ranges = nullptr; // device, ngroups + 1 entries (persistent kernel)
+  // per-size-bin launch: bin b uses binranges[first .. first+count]
+  std::vector<km_group_bin> bins;
+  int *binranges = nullptr; // device
 };
 
 namespace
@@ -551,6 +561,29 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
   if (o.precision != KM_PRECISION_FP64 || o.plan)
     return KM_ERROR_NOT_SUPPORTED;
   const bool exact = (o.flags & KM_FLAG_EXACT) != 0;
+  if (!(o.flags & KM_FLAG_PERSISTENT))
+  {
+    // one LDS-resident launch per size bin, each over its plan ranges
+    EntryArgs<T> a{};
+    a.Aptr = Aarray;
+    a.lda = 0;
+    a.pX = pX;
+    a.pY = pY;
+    a.batch = plan->batch;
+    a.order = plan->order;
+    a.owned = 1;
+    for (const km_group_bin &bn : plan->bins)
+    {
+      KernelInfo kb;
+      if (!km::lds_kernel<T>(plan->d, bn.n, exact, false, &kb))
+        return KM_ERROR_NOT_SUPPORTED;
+      a.ranges = plan->binranges + bn.first;
+      void *args[] = {&a};
+      KM_HIP(hipLaunchKernel(kb.fn, dim3(bn.count), dim3(kb.nt), args, 0,
+                             (hipStream_t)stream));
+    }
+    return KM_SUCCESS;
+  }
   KernelInfo k;
   int nmax = 0;
   if (!km::grouped_kernel<T>(plan->d, exact, &k, &nmax) || plan->nmax > nmax)
@@ -918,7 +951,52 @@ extern "C"
     ranges.push_back(batchCount);
     p->ngroups = (int)ranges.size() - 1;
     p->noutputs = noutputs;
+    // per-size bins for the default multi-launch path: each bin split into
+    // ranges of whole output runs, balanced inside the bin for the bin's
+    // LDS-resident kernel
+    std::vector<int> binranges;
+    for (int i = 0; i < batchCount;)
+    {
+      const int n = n_host[order[i]];
+      int j = i;
+      while (j < batchCount && n_host[order[j]] == n)
+        j++;
+      KernelInfo kb;
+      const bool have = (elem_bytes == 8)
+                            ? km::lds_kernel<double>(d, n, false, false, &kb)
+                            : km::lds_kernel<float>(d, n, false, false, &kb);
+      const long long cnt = j - i;
+      long long g = have ? (long long)device_cus() * occupancy(kb.fn, kb.nt) : 1;
+      g = std::max(1LL, std::min(g, cnt));
+      const long long tgt = (cnt + g - 1) / g;
+      km_group_bin bn;
+      bn.n = n;
+      bn.first = (int)binranges.size();
+      binranges.push_back(i);
+      long long in_group = 0;
+      for (int q = i; q < j;)
+      {
+        int r = q + 1;
+        while (r < j && ys[order[r]] == ys[order[q]])
+          r++;
+        if (in_group > 0 && in_group + (r - q) > tgt)
+        {
+          binranges.push_back(q);
+          in_group = 0;
+        }
+        in_group += r - q;
+        q = r;
+      }
+      binranges.push_back(j);
+      bn.count = (int)binranges.size() - bn.first - 1;
+      p->bins.push_back(bn);
+      // the next bin starts its own list (the shared boundary is repeated)
+      i = j;
+    }
     km_status st2 = [&]() -> km_status {
+      KM_HIP(hipMalloc(&p->binranges, sizeof(int) * binranges.size()));
+      KM_HIP(hipMemcpy(p->binranges, binranges.data(),
+                       sizeof(int) * binranges.size(), hipMemcpyHostToDevice));
       KM_HIP(hipMalloc(&p->nvec, sizeof(int) * batchCount));
       KM_HIP(hipMalloc(&p->order, sizeof(int) * batchCount));
       KM_HIP(hipMalloc(&p->ranges, sizeof(int) * ranges.size()));
@@ -949,6 +1027,8 @@ extern "C"
       (void)hipFree(plan->order);
     if (plan->ranges)
       (void)hipFree(plan->ranges);
+    if (plan->binranges)
+      (void)hipFree(plan->binranges);
     delete plan;
     return KM_SUCCESS;
   }

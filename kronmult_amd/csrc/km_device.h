@@ -299,7 +299,7 @@ __device__ __forceinline__ T prefetch_a(const EntryArgs<T> &a, int b, int tid)
 {
   constexpr int PER_LINE = 128 / (int)sizeof(T);
   T v = 0;
-  const int span = (N - 1) * a.lda + N; // elements of one matrix
+  const int span = (N - 1) * (a.lda ? a.lda : N) + N; // elements of one matrix
   const int lines = (span + PER_LINE - 1) / PER_LINE;
 #pragma unroll
   for (int k = 0; k < D; k++)

@@ -32,7 +32,7 @@ def _oracle(d, ns, yg, mats, X, Y0):
     return Y
 
 
-def _gpu(km, d, ns, yg, mats, X, Y0, exact=False, runs=1):
+def _gpu(km, d, ns, yg, mats, X, Y0, exact=False, runs=1, persistent=False):
     import torch
     dev = torch.device("cuda:0")
     batch = len(ns)
@@ -48,7 +48,7 @@ def _gpu(km, d, ns, yg, mats, X, Y0, exact=False, runs=1):
     for _ in range(runs):
         for n in tY:
             tY[n].copy_(torch.tensor(Y0[n], device=dev))
-        km.kronmult_grouped(plan, table, pX, pY, opt=km.options(exact=exact))
+        km.kronmult_grouped(plan, table, pX, pY, opt=km.options(exact=exact, persistent=persistent))
         torch.cuda.synchronize()
         outs.append({n: t.cpu().numpy() for n, t in tY.items()})
     info = plan.info()
@@ -65,10 +65,11 @@ def km():
 
 @pytest.mark.parametrize("d,sizes", [(4, [2, 3, 4, 5, 6, 7, 8]), (3, [1, 2, 5, 8]), (2, [3, 8]),
                                      (5, [2, 3, 6]), (6, [2, 3, 4]), (1, [4, 7])])
-def test_grouped_vs_oracle(km, d, sizes):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_grouped_vs_oracle(km, d, sizes, persistent):
     ns, yg, mats, X, Y0 = _make(d * 10 + len(sizes), d, sizes, 300, 5)
     want = _oracle(d, ns, yg, mats, X, Y0)
-    outs, (nout, ngroups, nmax) = _gpu(km, d, ns, yg, mats, X, Y0, runs=2)
+    outs, (nout, ngroups, nmax) = _gpu(km, d, ns, yg, mats, X, Y0, runs=2, persistent=persistent)
     assert nmax == max(ns)
     assert nout == len({(n, g) for n, g in zip(ns, yg)})
     for n in want:
@@ -88,9 +89,10 @@ def test_grouped_exact_single_entry_outputs_bitwise(km):
     for b, n in enumerate(ns):
         Y0.setdefault(n, np.zeros((len(ns), n ** d)))
     want = _oracle(d, ns, yg, mats, X, Y0)
-    outs, _ = _gpu(km, d, ns, yg, mats, X, Y0, exact=True)
-    for n in want:
-        assert np.array_equal(outs[0][n], want[n])
+    for persistent in (False, True):
+        outs, _ = _gpu(km, d, ns, yg, mats, X, Y0, exact=True, persistent=persistent)
+        for n in want:
+            assert np.array_equal(outs[0][n], want[n])
 
 
 def test_grouped_rejects_oversize(km):
