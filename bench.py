@@ -168,7 +168,7 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
 
-    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"{kernel_name}_d6n6_b4096_traffic.txt"))
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"{kernel_name}_d6n6_b4096.txt"))
 
     cpu = None
     if not args.no_cpu:
