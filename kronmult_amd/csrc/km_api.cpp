@@ -342,25 +342,7 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
     c.a.owned = 0;
   }
   hipError_t e;
-  if (onchip && k.split > 1)
-  {
-    // kron_slab_split: `split` workgroups per entry range, partners 8 ids
-    // apart (same XCD under round-robin dispatch)
-    int nranges;
-    if (r.plan)
-      nranges = r.plan->ngroups;
-    else
-    {
-      long long g = std::max(1LL, (long long)cus * occupancy(k.fn, k.nt) / k.split);
-      g = std::min<long long>(g, batch);
-      c.a.chunk = (int)((batch + g - 1) / g);
-      nranges = (batch + c.a.chunk - 1) / c.a.chunk;
-    }
-    const int gridx = ((nranges + 7) / 8) * 8 * k.split;
-    void *args[] = {&c.a, &nranges};
-    e = hipLaunchKernel(k.fn, dim3(gridx), dim3(k.nt), args, 0, stream);
-  }
-  else if (onchip)
+  if (onchip)
   {
     void *args[] = {&c.a};
     e = hipLaunchKernel(k.fn, dim3(grid), dim3(k.nt), args, 0, stream);
@@ -768,8 +750,7 @@ extern "C"
       G = std::max(1LL, (long long)device_cus() * occupancy(k2.fn, k2.nt) * 2 / gy);
     }
     else
-      G = std::max(1LL, (long long)device_cus() * occupancy(k.fn, k.nt) /
-                            (onchip ? k.split : 1));
+      G = (long long)device_cus() * occupancy(k.fn, k.nt);
     G = std::max(1LL, std::min<long long>(G, batchCount));
     const long long target = (batchCount + G - 1) / G;
     std::vector<int> ranges;

@@ -1076,174 +1076,6 @@ __global__ __launch_bounds__(NT) void kron_pair(EntryArgs<T> a, int e0, int e1)
 }
 
 // ---------------------------------------------------------------------------
-// kron_slab_split: the slab algorithm with the mode-2 rows split over H
-// workgroups.  Workgroup half h contracts mode 2 FIRST, restricted to rows
-// [h*N/H, (h+1)*N/H) of A_2, straight from X in global memory (coalesced over
-// the n^(d-2) mode-2 fibers, prefetched one slab ahead in registers), then
-// modes d..3 on that (N/H) x n^(d-2) tensor in LDS, then accumulates mode 1
-// into registers.  Its outputs are the contiguous blocks
-// Y(i1, h*N/H : (h+1)*N/H, :), so the H workgroups of an entry write disjoint
-// elements, and each holds 1/H of the accumulator -- which is what frees the
-// registers for a full workgroup of 648 threads.  The partner workgroups of an
-// entry are 8 workgroup ids apart, so round-robin dispatch places them on the
-// same XCD and the second read of X is served by that XCD's L2.
-// Mode order: 2, d, d-1, ..., 3, 1 (rounding differs from Alg. 993; the
-// KM_FLAG_EXACT path keeps kron_slab).
-// ---------------------------------------------------------------------------
-template<typename T, int D, int N, int NT, int H, bool EXACT, int MODE>
-__device__ __forceinline__ void split_stages(T *tb, const EntryArgs<T> &a,
-                                             int b, int tid)
-{
-  constexpr int TS = (N / H) * cpow(N, D - 2);
-  constexpr int F = TS / N;
-  constexpr int STRIDE = cpow(N, D - 1 - MODE);
-  stage_apply<T, N, NT, EXACT, F, STRIDE>(tb, a_of(a, b, MODE, D), a.lda, tid);
-  __syncthreads();
-  if constexpr (MODE > 2)
-    split_stages<T, D, N, NT, H, EXACT, MODE - 1>(tb, a, b, tid);
-}
-
-template<typename T, int D, int N, int NT, int H, bool EXACT>
-__global__ __launch_bounds__(NT) void kron_slab_split(EntryArgs<T> a,
-                                                      int nranges)
-{
-  using C = typename Acc<T>::type;
-  constexpr int SL = cpow(N, D - 1);     // one slab X(j1, :)
-  constexpr int M2 = cpow(N, D - 2);     // mode-2 fibers = positions r
-  constexpr int RH = N / H;              // A_2 rows per workgroup
-  constexpr int TS = RH * M2;            // LDS tensor (i2', j3..jd)
-  constexpr int F2 = (M2 + NT - 1) / NT; // mode-2 fibers per thread
-  constexpr int RA = (TS + NT - 1) / NT; // accumulator positions per i1
-  static_assert(N % H == 0, "H must divide n");
-  __shared__ T tb[TS];
-  const int tid = threadIdx.x;
-  const int w = blockIdx.x;
-  const int g = (w / (8 * H)) * 8 + (w % 8);
-  const int h = (w / 8) % H;
-  if (g >= nranges)
-    return;
-  int p0, p1;
-  if (a.ranges)
-  {
-    p0 = cld(a.ranges + g);
-    p1 = cld(a.ranges + g + 1);
-  }
-  else
-  {
-    p0 = g * a.chunk;
-    p1 = min(a.batch, p0 + a.chunk);
-  }
-  if (p0 >= p1)
-    return;
-
-  C acc[N][RA];
-  T *target = nullptr;
-  C xr[F2][N];
-  auto load_x = [&](const T *xp) {
-#pragma unroll
-    for (int f = 0; f < F2; f++)
-    {
-      const int r = tid + NT * f;
-      if (M2 % NT == 0 || r < M2)
-      {
-#pragma unroll
-        for (int j = 0; j < N; j++)
-          xr[f][j] = static_cast<C>(gld(xp + j * M2 + r));
-      }
-    }
-  };
-  load_x(x_of(a, entry_at(a, p0)));
-
-  const int nslab = (p1 - p0) * N;
-  for (int t = 0; t < nslab; ++t)
-  {
-    const int p = p0 + t / N;
-    const int j1 = t - (t / N) * N;
-    const int b = entry_at(a, p);
-    if (j1 == 0)
-    {
-      T *yb = y_of(a, b);
-      if (yb != target)
-      {
-        if (target)
-        {
-#pragma unroll
-          for (int i1 = 0; i1 < N; i1++)
-            flush_run<RA, NT, TS, 2>(target + i1 * SL + h * TS, acc[i1], tid,
-                                     a.owned);
-        }
-#pragma unroll
-        for (int i1 = 0; i1 < N; i1++)
-#pragma unroll
-          for (int k = 0; k < RA; k++)
-            acc[i1][k] = 0;
-        target = yb;
-      }
-    }
-    // mode 2, rows [h*RH, h*RH + RH) of A_2, from the prefetched slab
-    {
-      const T *A2 = a_of(a, b, 1, D);
-      C a2[RH][N];
-#pragma unroll
-      for (int j = 0; j < N; j++)
-#pragma unroll
-        for (int i = 0; i < RH; i++)
-          a2[i][j] = static_cast<C>(cld(A2 + (h * RH + i + j * a.lda)));
-#pragma unroll
-      for (int f = 0; f < F2; f++)
-      {
-        const int r = tid + NT * f;
-        if (M2 % NT == 0 || r < M2)
-        {
-#pragma unroll
-          for (int i = 0; i < RH; i++)
-          {
-            C o = a2[i][0] * xr[f][0];
-#pragma unroll
-            for (int j = 1; j < N; j++)
-              o = madd<false>(o, a2[i][j], xr[f][j]);
-            tb[i * M2 + r] = static_cast<T>(o);
-          }
-        }
-      }
-    }
-    // next slab's X is in flight during the LDS passes
-    if (t + 1 < nslab)
-    {
-      const int pn = p0 + (t + 1) / N;
-      const int jn = (t + 1) - ((t + 1) / N) * N;
-      load_x(x_of(a, entry_at(a, pn)) + (long long)jn * SL);
-    }
-    __syncthreads();
-    split_stages<T, D, N, NT, H, EXACT, D - 1>(tb, a, b, tid);
-    // mode 1: acc(i1, q) += A_1(i1, j1) * tb(q)
-    {
-      const T *A1 = a_of(a, b, 0, D);
-      C a1[N];
-#pragma unroll
-      for (int i1 = 0; i1 < N; i1++)
-        a1[i1] = static_cast<C>(cld(A1 + (i1 + j1 * a.lda)));
-#pragma unroll
-      for (int k = 0; k < RA; k++)
-      {
-        const int q = tid + NT * k;
-        if (TS % NT == 0 || q < TS)
-        {
-          const C z = tb[q];
-#pragma unroll
-          for (int i1 = 0; i1 < N; i1++)
-            acc[i1][k] = madd<false>(acc[i1][k], a1[i1], z);
-        }
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i1 = 0; i1 < N; i1++)
-    flush_run<RA, NT, TS, 2>(target + i1 * SL + h * TS, acc[i1], tid, a.owned);
-}
-
-// ---------------------------------------------------------------------------
 // kron_global: any d <= 6 and any (rectangular) m_k x n_k with
 // n_k <= KM_GLOBAL_MAX_N; one workgroup per entry; stages go through the caller's workspace as in the reference
 // (kronm_backward.hpp:92-139).  Square shapes update one buffer in place;

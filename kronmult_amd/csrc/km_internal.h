@@ -23,7 +23,6 @@ struct KernelInfo
   const void *fn = nullptr; // __global__ function
   int nt = 0;               // threads per workgroup
   int family = FAM_LDS;
-  int split = 1;            // kron_slab_split: workgroups per entry
 };
 
 // threads per workgroup of kron_lds for a tensor of s elements

@@ -22,15 +22,6 @@ namespace km
 template<>
 bool slab_kernel<double>(int d, int n, bool exact, KernelInfo *out)
 {
-  // fast path: mode-2 rows split over two workgroups (FMA rounding only)
-  if (d == 6 && n == 6 && !exact)
-  {
-    out->fn = (const void *)&kron_slab_split<double, 6, 6, 648, 2, false>;
-    out->nt = 648;
-    out->family = FAM_SLAB;
-    out->split = 2;
-    return true;
-  }
   KM_SLAB_CASE(double, 6, 6, KM_SLAB_NT)
   KM_SLAB_CASE(double, 5, 8, 512)
   return false;
