@@ -172,6 +172,22 @@ km_status km_kronmult_vbatched_f32(int d, const int *m, const int *n,
  * ------------------------------------------------------------------------ */
 km_status km_plan_create(km_plan *plan, int d, int n, int elem_bytes,
                          const void *const *pY, int batchCount, void *stream);
+
+/* Plan orders.  KM_ORDER_BY_OUTPUT is km_plan_create's (owned outputs,
+ * deterministic).  KM_ORDER_BY_INPUT sorts entries by their X pointer so that
+ * the entries reading the same input run back to back (the repeated reads
+ * hit L2: pooled-input batches such as sparse-grid operator applies); the
+ * outputs are then shared between workgroups and flushed with atomics, so
+ * the result is not bitwise reproducible. */
+typedef enum km_plan_order
+{
+  KM_ORDER_BY_OUTPUT = 0,
+  KM_ORDER_BY_INPUT = 1
+} km_plan_order;
+
+km_status km_plan_create_ordered(km_plan *plan, int d, int n, int elem_bytes,
+                                 const void *const *pX, const void *const *pY,
+                                 int batchCount, int order_by, void *stream);
 km_status km_plan_destroy(km_plan plan);
 /* number of distinct outputs and of workgroups in the plan */
 km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);

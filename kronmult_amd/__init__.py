@@ -56,7 +56,7 @@ C_ABI_SYMBOLS = (
     "km_kronmult_pbatched_f64", "km_kronmult_pbatched_f32",
     "km_kronmult_xbatched_f64", "km_kronmult_xbatched_f32",
     "km_kronmult_vbatched_f64", "km_kronmult_vbatched_f32",
-    "km_plan_create", "km_plan_destroy", "km_plan_info",
+    "km_plan_create", "km_plan_create_ordered", "km_plan_destroy", "km_plan_info",
     "km_query_kernel", "km_kron_flops",
     "km_group_plan_create", "km_group_plan_destroy", "km_group_plan_info",
     "km_kronmult_grouped_f64", "km_kronmult_grouped_f32",
@@ -133,6 +133,8 @@ def lib():
             f.restype = _I
         L.km_plan_create.argtypes = [C.POINTER(C.c_void_p), _I, _I, _I, _P, _I, _P]
         L.km_plan_create.restype = _I
+        L.km_plan_create_ordered.argtypes = [C.POINTER(C.c_void_p), _I, _I, _I, _P, _P, _I, _I, _P]
+        L.km_plan_create_ordered.restype = _I
         L.km_plan_destroy.argtypes = [_P]
         L.km_plan_destroy.restype = _I
         L.km_plan_info.argtypes = [_P, C.POINTER(_I), C.POINTER(_I)]
@@ -264,13 +266,23 @@ def kronmult_vbatched(m: Sequence[int], n: Sequence[int], Aarray, pX, pY, W, Wca
 # ---------------------------------------------------------------------------
 # plans (deterministic output-sorted reduction)
 # ---------------------------------------------------------------------------
-class Plan:
-    """Deterministic reduction plan for one output pointer array ``pY``."""
+ORDER_BY_OUTPUT, ORDER_BY_INPUT = 0, 1
 
-    def __init__(self, d: int, n: int, elem_bytes: int, pY, batchCount: int, stream=None):
+
+class Plan:
+    """Reduction plan for one output pointer array ``pY``: by default the
+    deterministic output-ordered plan; ``order=ORDER_BY_INPUT`` (with ``pX``)
+    orders the entries by input instead (atomic flush)."""
+
+    def __init__(self, d: int, n: int, elem_bytes: int, pY, batchCount: int, stream=None,
+                 order: int = ORDER_BY_OUTPUT, pX=None):
         h = C.c_void_p()
-        _check(lib().km_plan_create(C.byref(h), d, n, elem_bytes, _addr(pY), batchCount, _stream(stream)),
-               "km_plan_create")
+        if order == ORDER_BY_OUTPUT and pX is None:
+            st = lib().km_plan_create(C.byref(h), d, n, elem_bytes, _addr(pY), batchCount, _stream(stream))
+        else:
+            st = lib().km_plan_create_ordered(C.byref(h), d, n, elem_bytes, _addr(pX), _addr(pY), batchCount,
+                                              order, _stream(stream))
+        _check(st, "km_plan_create")
         self.handle = h.value
 
     def info(self):

@@ -29,6 +29,7 @@ struct km_plan_s
   int device = 0;
   int ngroups = 0;
   int noutputs = 0;
+  int owned = 1; // 1: output-ordered (owned outputs); 0: input-ordered (atomics)
   int *order = nullptr;  // device, batch entries
   int *ranges = nullptr; // device, ngroups + 1 entries
 };
@@ -260,7 +261,7 @@ km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
     if (r.plan)
     {
       c.a.ranges = r.plan->ranges;
-      c.a.owned = 1;
+      c.a.owned = r.plan->owned;
       gx = r.plan->ngroups;
     }
     else
@@ -326,7 +327,7 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
       return KM_ERROR_INSUFFICIENT_WORKSPACE;
     c.a.ranges = p->ranges;
     c.a.order = p->order;
-    c.a.owned = 1;
+    c.a.owned = p->owned;
   }
   else
   {
@@ -714,22 +715,48 @@ extern "C"
                            const void *const *pY, int batchCount,
                            void *stream)
   {
+    return km_plan_create_ordered(plan, d, n, elem_bytes, nullptr, pY,
+                                  batchCount, KM_ORDER_BY_OUTPUT, stream);
+  }
+
+  km_status km_plan_create_ordered(km_plan *plan, int d, int n, int elem_bytes,
+                                   const void *const *pX,
+                                   const void *const *pY, int batchCount,
+                                   int order_by, void *stream)
+  {
     if (!plan || (elem_bytes != 4 && elem_bytes != 8) || batchCount < 1 ||
-        !pY)
+        !pY ||
+        (order_by != KM_ORDER_BY_OUTPUT && order_by != KM_ORDER_BY_INPUT) ||
+        (order_by == KM_ORDER_BY_INPUT && !pX))
       return KM_ERROR_INVALID_VALUE;
     *plan = nullptr;
     km_status s = check_dn(d, n, batchCount);
     if (s != KM_SUCCESS)
       return s;
     hipStream_t st = (hipStream_t)stream;
-    std::vector<uintptr_t> ys(batchCount);
+    std::vector<uintptr_t> ys(batchCount), xs;
     KM_HIP(hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
                           hipMemcpyDeviceToHost, st));
+    if (order_by == KM_ORDER_BY_INPUT)
+    {
+      xs.resize(batchCount);
+      KM_HIP(hipMemcpyAsync(xs.data(), pX, sizeof(void *) * batchCount,
+                            hipMemcpyDeviceToHost, st));
+    }
     KM_HIP(hipStreamSynchronize(st));
     std::vector<int> order(batchCount);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return ys[x] < ys[y]; });
+    if (order_by == KM_ORDER_BY_INPUT)
+    {
+      // entries sharing an input become neighbours (the repeated X reads hit
+      // L2); ties keep output order so equal outputs still merge on chip
+      std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return xs[x] != xs[y] ? xs[x] < xs[y] : ys[x] < ys[y];
+      });
+    }
+    else
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int x, int y) { return ys[x] < ys[y]; });
     // workgroup count the default kernel would use
     Resolved r;
     KernelInfo k;
@@ -756,23 +783,36 @@ extern "C"
     std::vector<int> ranges;
     ranges.push_back(0);
     int noutputs = 0;
-    long long in_group = 0;
-    for (int i = 0; i < batchCount;)
+    if (order_by == KM_ORDER_BY_INPUT)
     {
-      int j = i + 1;
-      while (j < batchCount && ys[order[j]] == ys[order[i]])
-        j++;
-      noutputs++;
-      if (in_group > 0 && in_group + (j - i) > target)
+      // outputs are shared between workgroups (atomic flush): even split
+      std::vector<uintptr_t> u(ys);
+      std::sort(u.begin(), u.end());
+      noutputs = (int)(std::unique(u.begin(), u.end()) - u.begin());
+      for (long long i = target; i < batchCount; i += target)
+        ranges.push_back((int)i);
+    }
+    else
+    {
+      long long in_group = 0;
+      for (int i = 0; i < batchCount;)
       {
-        ranges.push_back(i);
-        in_group = 0;
+        int j = i + 1;
+        while (j < batchCount && ys[order[j]] == ys[order[i]])
+          j++;
+        noutputs++;
+        if (in_group > 0 && in_group + (j - i) > target)
+        {
+          ranges.push_back(i);
+          in_group = 0;
+        }
+        in_group += j - i;
+        i = j;
       }
-      in_group += j - i;
-      i = j;
     }
     ranges.push_back(batchCount);
     km_plan p = new km_plan_s;
+    p->owned = (order_by == KM_ORDER_BY_OUTPUT) ? 1 : 0;
     p->d = d;
     p->n = n;
     p->elem_bytes = elem_bytes;

@@ -345,3 +345,64 @@ def test_twopass_plan_deterministic(km):
     want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
     assert rel_err(g1, want) <= RTOL_FP64
     plan.close()
+
+
+def test_input_ordered_plan_pooled_xbatched(km):
+    """Input-ordered plan (entries sharing an X become neighbours, atomic
+    flush) on pooled xbatched inputs."""
+    import torch
+    rng = np.random.default_rng(123)
+    d, n, batch, P, nX, nY, lda = 6, 4, 200, 7, 11, 9, 4
+    pools = [rng.uniform(-1, 1, (P, n, lda)) for _ in range(d)]
+    aidx = rng.integers(0, P, (batch, d))
+    X = rng.uniform(-1, 1, (nX, n ** d))
+    xidx = rng.integers(0, nX, batch)
+    Y0 = rng.uniform(-1, 1, (nY, n ** d))
+    yidx = rng.integers(0, nY, batch)
+    want = O.xbatched(pools, aidx, X, xidx, Y0, yidx, variant=O.BACKWARD)
+    dev = torch.device("cuda:0")
+    tpools = [torch.tensor(p, device=dev) for p in pools]
+    table = km.matrix_pointer_table(tpools, aidx)
+    tX = torch.tensor(X, device=dev)
+    tY = torch.tensor(Y0, device=dev)
+    pX = km.pointer_array(tX, xidx)
+    pY = km.pointer_array(tY, yidx)
+    plan = km.Plan(d, n, 8, pY, batch, order=km.ORDER_BY_INPUT, pX=pX)
+    nout, _ = plan.info()
+    assert nout == len(set(yidx.tolist()))
+    km.kronmult_xbatched(d, n, table, lda, pX, pY, None, batch, opt=km.options(plan=plan))
+    torch.cuda.synchronize()
+    assert rel_err(tY.cpu().numpy(), want) <= RTOL_FP64
+    plan.close()
+
+
+@pytest.mark.parametrize("d,n", [(3, 8), (6, 6)])
+def test_graph_capture_replays(km, d, n):
+    """A pbatched call captured in a HIP graph (through torch.cuda.graph) and
+    replayed several times, with eager work in between, matches the eager
+    call every time."""
+    import torch
+    dev = torch.device("cuda:0")
+    batch = 64 if n ** d < 5000 else 16
+    A, X, Y0 = rng_inputs(900 + d * n, d, n, batch, batch // 4)
+    tA = torch.tensor(A, device=dev)
+    tX = torch.tensor(X, device=dev)
+    tY0 = torch.tensor(Y0, device=dev)
+    tY = tY0.clone()
+    pX = km.pointer_array(tX, np.arange(batch))
+    pY = km.pointer_array(tY, np.arange(batch) // 4)
+    km.kronmult_pbatched(d, n, tA, pX, pY, None, batch)  # eager (also warms caches)
+    torch.cuda.synchronize()
+    eager = (tY - tY0).clone()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        km.kronmult_pbatched(d, n, tA, pX, pY, None, batch)
+    for rep in range(4):
+        tY.copy_(tY0)
+        junk = torch.ones(1 << 20, device=dev, dtype=torch.float64).sum()  # eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        assert junk.item() == float(1 << 20)
+        assert torch.allclose(tY - tY0, eager, rtol=0, atol=1e-12 * float(eager.abs().max())), rep
+    want = O.pbatched(A, X, Y0, np.arange(batch) // 4, variant=O.BACKWARD)
+    assert rel_err(tY.cpu().numpy(), want) <= RTOL_FP64
