@@ -69,7 +69,11 @@ typedef enum km_precision
                                      default uses fused multiply-add          */
 #define KM_FLAG_FORCE_GLOBAL 0x2u /* always use the workspace-staged kernel   */
 #define KM_FLAG_PERSISTENT 0x4u   /* grouped calls: one persistent launch over
-                                     all sizes instead of one launch per size */
+                                     all sizes instead of one launch per size;
+                                     outputs owned (as DETERMINISTIC)         */
+#define KM_FLAG_DETERMINISTIC 0x8u /* grouped calls: every output owned by one
+                                      workgroup, no atomics, bitwise
+                                      reproducible run to run                */
 
 typedef struct km_plan_s *km_plan;
 
@@ -203,12 +207,16 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
  *   pX/pY:  batchCount device pointers to vectors of n_b^d elements
  *           (outputs may repeat; entries sharing an output must have the
  *           same n_b).
- * The group plan bins the entries by size, groups them by output and splits
- * them over workgroups; each output is owned by one workgroup
- * (deterministic, no atomics).  By default a call issues one LDS-resident
- * launch per size bin (back to back on `stream`, capturable in a HIP graph);
- * KM_FLAG_PERSISTENT runs all bins in a single persistent launch instead.  It reads the host array n_host and
- * the device pointer array pY once; calls must pass the same pY contents.
+ * The group plan bins the entries by size and sorts each bin by output.  By
+ * default a call issues one on-chip launch per size bin (back to back on
+ * `stream`, capturable in a HIP graph), each bin split evenly over the
+ * resident workgroups, outputs shared between workgroups accumulated with
+ * atomics.  KM_FLAG_DETERMINISTIC splits each bin at output boundaries
+ * instead, so every output is owned by one workgroup (no atomics, bitwise
+ * reproducible); KM_FLAG_PERSISTENT runs all bins in a single persistent
+ * launch with owned outputs.  Creating the plan reads the host array n_host
+ * and the device pointer array pY once; calls must pass the same pY
+ * contents.
  * Supported sizes: n_b <= 8 for d <= 4, n_b <= 6 for d = 5, n_b <= 4 for
  * d = 6 (KM_ERROR_NOT_SUPPORTED otherwise).
  * ------------------------------------------------------------------------ */

@@ -38,6 +38,9 @@ struct km_group_bin
 {
   int n;       // entry size of the bin
   int first;   // first range of the bin in binranges
+  int start;   // first position of the bin in the plan's order
+  int cnt;     // entries in the bin
+  int chunk;   // entries per workgroup of the even (atomic) split
   int count;   // ranges of the bin
 };
 
@@ -129,6 +132,32 @@ long long ipow(int b, int e)
   return p;
 }
 
+// Split positions [starts.front(), starts.back()) into at most G ranges that
+// end only at run boundaries.  starts holds the first position of every run
+// followed by the end position; pre[i] is the cost of the positions before
+// starts[i] (pre[0] = 0, pre.back() = total).  Each cut is the boundary whose
+// prefix cost is nearest k*total/G, so one resident wave of G workgroups
+// covers all ranges and each range is within one run of the ideal cost.
+// Appends the cuts and the end position to out (which ends with the start).
+void balanced_cuts(const std::vector<int> &starts, const std::vector<double> &pre,
+                   long long G, std::vector<int> &out)
+{
+  const double total = pre.back();
+  for (long long k = 1; k < G; k++)
+  {
+    const double ideal = total * (double)k / (double)G;
+    size_t idx = std::lower_bound(pre.begin(), pre.end(), ideal) - pre.begin();
+    if (idx >= pre.size())
+      idx = pre.size() - 1;
+    if (idx > 0 && ideal - pre[idx - 1] < pre[idx] - ideal)
+      idx--;
+    const int cut = starts[idx];
+    if (cut > out.back() && cut < starts.back())
+      out.push_back(cut);
+  }
+  out.push_back(starts.back());
+}
+
 // kronm.hpp:37-89: forward only when strictly cheaper than backward
 bool alg993_forward(int d, const int *m, const int *n)
 {
@@ -192,11 +221,24 @@ bool pick_onchip(int d, int n, const Resolved &r, KernelInfo *k)
   // without EXACT the contraction order only changes rounding, so the
   // forward request may use any kernel
   bool fwd = r.fwd && r.exact;
+  // the register-blocked kernels contract in the forward order
+  if (KM_USE_REG && (fwd || !r.exact) && km::reg_kernel<T>(d, n, r.exact, k))
+    return true;
   if (km::lds_kernel<T>(d, n, r.exact, fwd, k))
     return true;
   if (!fwd && km::slab_kernel<T>(d, n, r.exact, k))
     return true;
   return false;
+}
+
+// On-chip kernel of one size bin of a group plan (backward order when EXACT).
+template<typename T>
+bool bin_kernel(int d, int n, bool exact, KernelInfo *k)
+{
+  Resolved r;
+  r.exact = exact;
+  r.fwd = false;
+  return pick_onchip<T>(d, n, r, k) && k->family == km::FAM_LDS;
 }
 
 // Launch description common to all entry points.
@@ -570,17 +612,36 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     a.lda = 0;
     a.pX = pX;
     a.pY = pY;
-    a.batch = plan->batch;
-    a.order = plan->order;
-    a.owned = 1;
+    const bool owned = (o.flags & KM_FLAG_DETERMINISTIC) != 0;
     for (const km_group_bin &bn : plan->bins)
     {
       KernelInfo kb;
-      if (!km::lds_kernel<T>(plan->d, bn.n, exact, false, &kb))
+      if (!bin_kernel<T>(plan->d, bn.n, exact, &kb))
         return KM_ERROR_NOT_SUPPORTED;
-      a.ranges = plan->binranges + bn.first;
+      int grid;
+      if (owned)
+      {
+        // ranges of whole output runs, positions into the full order
+        a.batch = plan->batch;
+        a.order = plan->order;
+        a.ranges = plan->binranges + bn.first;
+        a.chunk = 0;
+        a.owned = 1;
+        grid = bn.count;
+      }
+      else
+      {
+        // even split of the bin's positions; equal outputs stay adjacent
+        // (merged on chip), the run shared by two workgroups uses atomics
+        a.batch = bn.cnt;
+        a.order = plan->order + bn.start;
+        a.ranges = nullptr;
+        a.chunk = bn.chunk;
+        a.owned = 0;
+        grid = (bn.cnt + bn.chunk - 1) / bn.chunk;
+      }
       void *args[] = {&a};
-      KM_HIP(hipLaunchKernel(kb.fn, dim3(bn.count), dim3(kb.nt), args, 0,
+      KM_HIP(hipLaunchKernel(kb.fn, dim3(grid), dim3(kb.nt), args, 0,
                              (hipStream_t)stream));
     }
     return KM_SUCCESS;
@@ -791,26 +852,27 @@ extern "C"
       noutputs = (int)(std::unique(u.begin(), u.end()) - u.begin());
       for (long long i = target; i < batchCount; i += target)
         ranges.push_back((int)i);
+      ranges.push_back(batchCount);
     }
     else
     {
-      long long in_group = 0;
+      // ranges end only where one output's run of entries ends
+      std::vector<int> starts;
+      std::vector<double> pre;
       for (int i = 0; i < batchCount;)
       {
         int j = i + 1;
         while (j < batchCount && ys[order[j]] == ys[order[i]])
           j++;
-        noutputs++;
-        if (in_group > 0 && in_group + (j - i) > target)
-        {
-          ranges.push_back(i);
-          in_group = 0;
-        }
-        in_group += j - i;
+        starts.push_back(i);
+        pre.push_back((double)i);
         i = j;
       }
+      noutputs = (int)starts.size();
+      starts.push_back(batchCount);
+      pre.push_back((double)batchCount);
+      balanced_cuts(starts, pre, G, ranges);
     }
-    ranges.push_back(batchCount);
     km_plan p = new km_plan_s;
     p->owned = (order_by == KM_ORDER_BY_OUTPUT) ? 1 : 0;
     p->d = d;
@@ -959,36 +1021,34 @@ extern "C"
       const double c = 2.0 * d * sz * n_host[b] + 4096.0;
       return (waves > 1 && sz <= small) ? c / waves : c;
     };
-    double total = 0;
-    for (int b = 0; b < batchCount; b++)
-      total += cost(b);
     const long long G = std::max(
         1LL, std::min<long long>((long long)device_cus() * occupancy(k.fn, k.nt),
                                  batchCount));
-    const double target = total / G;
     std::vector<int> ranges;
     ranges.push_back(0);
-    double in_group = 0;
     int noutputs = 0;
-    for (int i = 0; i < batchCount;)
     {
-      int j = i + 1;
-      while (j < batchCount && ys[order[j]] == ys[order[i]] &&
-             n_host[order[j]] == n_host[order[i]])
-        j++;
-      double c = 0;
-      for (int q = i; q < j; q++)
-        c += cost(order[q]);
-      noutputs++;
-      if (in_group > 0 && in_group + c > target)
+      // runs: entries of one output and one size
+      std::vector<int> starts;
+      std::vector<double> pre;
+      double acc = 0;
+      for (int i = 0; i < batchCount;)
       {
-        ranges.push_back(i);
-        in_group = 0;
+        int j = i + 1;
+        while (j < batchCount && ys[order[j]] == ys[order[i]] &&
+               n_host[order[j]] == n_host[order[i]])
+          j++;
+        starts.push_back(i);
+        pre.push_back(acc);
+        for (int q = i; q < j; q++)
+          acc += cost(order[q]);
+        noutputs++;
+        i = j;
       }
-      in_group += c;
-      i = j;
+      starts.push_back(batchCount);
+      pre.push_back(acc);
+      balanced_cuts(starts, pre, G, ranges);
     }
-    ranges.push_back(batchCount);
     p->ngroups = (int)ranges.size() - 1;
     p->noutputs = noutputs;
     // per-size bins for the default multi-launch path: each bin split into
@@ -1002,32 +1062,32 @@ extern "C"
       while (j < batchCount && n_host[order[j]] == n)
         j++;
       KernelInfo kb;
-      const bool have = (elem_bytes == 8)
-                            ? km::lds_kernel<double>(d, n, false, false, &kb)
-                            : km::lds_kernel<float>(d, n, false, false, &kb);
+      const bool have = (elem_bytes == 8) ? bin_kernel<double>(d, n, false, &kb)
+                                          : bin_kernel<float>(d, n, false, &kb);
       const long long cnt = j - i;
       long long g = have ? (long long)device_cus() * occupancy(kb.fn, kb.nt) : 1;
       g = std::max(1LL, std::min(g, cnt));
-      const long long tgt = (cnt + g - 1) / g;
       km_group_bin bn;
       bn.n = n;
+      bn.start = i;
+      bn.cnt = (int)cnt;
+      bn.chunk = (int)((cnt + g - 1) / g);
       bn.first = (int)binranges.size();
       binranges.push_back(i);
-      long long in_group = 0;
+      std::vector<int> starts;
+      std::vector<double> pre;
       for (int q = i; q < j;)
       {
         int r = q + 1;
         while (r < j && ys[order[r]] == ys[order[q]])
           r++;
-        if (in_group > 0 && in_group + (r - q) > tgt)
-        {
-          binranges.push_back(q);
-          in_group = 0;
-        }
-        in_group += r - q;
+        starts.push_back(q);
+        pre.push_back((double)(q - i));
         q = r;
       }
-      binranges.push_back(j);
+      starts.push_back(j);
+      pre.push_back((double)(j - i));
+      balanced_cuts(starts, pre, g, binranges);
       bn.count = (int)binranges.size() - bn.first - 1;
       p->bins.push_back(bn);
       // the next bin starts its own list (the shared boundary is repeated)

@@ -41,6 +41,11 @@ namespace km
 #endif
 constexpr int kLdsUnroll = KM_LDS_UNROLL;
 constexpr int kSlabUnroll = KM_SLAB_UNROLL;
+// kron_reg loads the next entry's X into registers during the current one
+#ifndef KM_REG_PREFETCH_X
+#define KM_REG_PREFETCH_X 0
+#endif
+constexpr bool kRegPrefetchX = KM_REG_PREFETCH_X != 0;
 
 // ---------------------------------------------------------------------------
 // arguments
@@ -442,6 +447,226 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
     keep_alive(pf);
   }
   flush_run<R, NT, S, 8>(target, acc, tid, a.owned);
+}
+
+// ---------------------------------------------------------------------------
+// kron_reg: register-blocked passes for n a power of two.  The n^d tensor of
+// an entry is held by NTH = n^(d-RM) threads with RR = n^RM elements each.
+// Pass g keeps the modes [g*RM, (g+1)*RM) in registers and contracts them
+// there (A_k in SGPRs, RR/n independent fibers per mode); between passes the
+// tensor is transposed through a padded LDS buffer (one pad element every
+// RR), so there are d/RM - 1 LDS round trips per entry instead of d.  Modes
+// run 1..d (the recursive / forward order, kronmultv_recur.hpp:90-145).
+// Layout g: thread t = (hi, lo) with hi the thread modes before the group and
+// lo the L_g = n^(d-(g+1)RM) modes after it; register r = the group's modes;
+// element index = hi*RR*L_g + r*L_g + lo.  Layout 0 makes every X load of a
+// wavefront contiguous; outputs accumulate in the last layout and are
+// transposed once per output run for a coalesced flush.
+// ---------------------------------------------------------------------------
+template<typename T, int D, int N, int RM, bool EXACT, int M>
+__device__ __forceinline__ void reg_group(typename Acc<T>::type (&v)[cpow(N, RM)],
+                                          const EntryArgs<T> &a, int b, int g)
+{
+  // contract group mode M (0 = slowest of the group): register stride N^(RM-1-M)
+  using C = typename Acc<T>::type;
+  constexpr int RR = cpow(N, RM);
+  constexpr int ST = cpow(N, RM - 1 - M);
+  constexpr int RB = (N <= 4) ? N : N / 2; // A rows held in SGPRs at a time
+  const T *Ak = a_of(a, b, g * RM + M, D);
+  const int lda = a.lda ? a.lda : N;
+  if constexpr (RB == N)
+  {
+    // all of A_k fits in SGPRs: contract each fiber in place
+    C am[N][N];
+    load_a<N, T>(am, Ak, lda);
+#pragma unroll
+    for (int base = 0; base < RR; base++)
+    {
+      if ((base / ST) % N != 0)
+        continue;
+      C x[N];
+#pragma unroll
+      for (int j = 0; j < N; j++)
+        x[j] = v[base + j * ST];
+#pragma unroll
+      for (int i = 0; i < N; i++)
+      {
+        C acc;
+        if constexpr (EXACT)
+        {
+          acc = 0;
+#pragma unroll
+          for (int j = 0; j < N; j++)
+            acc = madd<true>(acc, am[i][j], x[j]);
+        }
+        else
+        {
+          acc = am[i][0] * x[0];
+#pragma unroll
+          for (int j = 1; j < N; j++)
+            acc = madd<false>(acc, am[i][j], x[j]);
+        }
+        v[base + i * ST] = static_cast<C>(static_cast<T>(acc));
+      }
+    }
+    if constexpr (M + 1 < RM)
+      reg_group<T, D, N, RM, EXACT, M + 1>(v, a, b, g);
+    return;
+  }
+  C o[RR];
+#pragma unroll
+  for (int r0 = 0; r0 < N; r0 += RB)
+  {
+    C am[RB][N];
+#pragma unroll
+    for (int j = 0; j < N; j++)
+#pragma unroll
+      for (int i = 0; i < RB; i++)
+        am[i][j] = static_cast<C>(cld(Ak + (r0 + i + j * lda)));
+#pragma unroll
+    for (int base = 0; base < RR; base++)
+    {
+      if ((base / ST) % N != 0)
+        continue; // fiber start: group digit M is zero
+#pragma unroll
+      for (int i = 0; i < RB; i++)
+      {
+        C acc;
+        if constexpr (EXACT)
+        {
+          acc = 0;
+#pragma unroll
+          for (int j = 0; j < N; j++)
+            acc = madd<true>(acc, am[i][j], v[base + j * ST]);
+        }
+        else
+        {
+          acc = am[i][0] * v[base];
+#pragma unroll
+          for (int j = 1; j < N; j++)
+            acc = madd<false>(acc, am[i][j], v[base + j * ST]);
+        }
+        o[base + (r0 + i) * ST] = acc;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RR; r++)
+    v[r] = static_cast<C>(static_cast<T>(o[r]));
+  if constexpr (M + 1 < RM)
+    reg_group<T, D, N, RM, EXACT, M + 1>(v, a, b, g);
+}
+
+template<int D, int N, int RM>
+__device__ __forceinline__ int reg_index(int g, int t, int r)
+{
+  constexpr int RR = cpow(N, RM);
+  // L_g = N^(D - (g+1)*RM), computed without a runtime power
+  int L = 1;
+#pragma unroll
+  for (int k = 0; k < D; k++)
+    if (k < D - (g + 1) * RM)
+      L *= N;
+  return (t / L) * RR * L + r * L + (t % L);
+}
+
+template<int RR>
+__device__ __forceinline__ int reg_pad(int idx)
+{
+  return idx + idx / RR;
+}
+
+template<typename T, int D, int N, int RM, bool EXACT>
+__global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
+{
+  using C = typename Acc<T>::type;
+  constexpr int S = cpow(N, D);
+  constexpr int RR = cpow(N, RM);
+  constexpr int NTH = S / RR;
+  constexpr int NG = D / RM;
+  static_assert(D % RM == 0, "mode groups must tile d");
+  static_assert(NTH % 64 == 0, "whole wavefronts");
+  __shared__ T buf[S + S / RR];
+  const int tid = threadIdx.x;
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+
+  C v[RR];
+  C xr[RR];
+  T acc[RR];
+  T *target = nullptr;
+
+  auto load_x = [&](const T *xp) {
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      xr[r] = static_cast<C>(gld(xp + r * NTH + tid)); // layout 0: coalesced
+  };
+  // acc (last layout) -> LDS -> coalesced positions tid + NTH*k -> Y
+  auto flush = [&](T *dst) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      buf[reg_pad<RR>(reg_index<D, N, RM>(NG - 1, tid, r))] = acc[r];
+    __syncthreads();
+    T out[RR];
+#pragma unroll
+    for (int k = 0; k < RR; k++)
+      out[k] = buf[reg_pad<RR>(tid + NTH * k)];
+    flush_run<RR, NTH, S, RR>(dst, out, tid, a.owned);
+  };
+
+  if constexpr (kRegPrefetchX)
+    load_x(x_of(a, entry_at(a, p0)));
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    T *yb = y_of(a, b);
+    if (yb != target)
+    {
+      if (target)
+        flush(target);
+#pragma unroll
+      for (int r = 0; r < RR; r++)
+        acc[r] = 0;
+      target = yb;
+    }
+    if constexpr (!kRegPrefetchX)
+      load_x(x_of(a, b));
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      v[r] = xr[r];
+    T pf = 0;
+    if (p + 1 < p1)
+    {
+      const int bn = entry_at(a, p + 1);
+      if constexpr (kRegPrefetchX)
+        load_x(x_of(a, bn));
+      pf = prefetch_a<D, N>(a, bn, tid);
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+    {
+      reg_group<T, D, N, RM, EXACT, 0>(v, a, b, g);
+      if (g + 1 < NG)
+      {
+        __syncthreads(); // previous readers of buf are done
+#pragma unroll
+        for (int r = 0; r < RR; r++)
+          buf[reg_pad<RR>(reg_index<D, N, RM>(g, tid, r))] = static_cast<T>(v[r]);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RR; r++)
+          v[r] = static_cast<C>(buf[reg_pad<RR>(reg_index<D, N, RM>(g + 1, tid, r))]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      acc[r] = acc[r] + static_cast<T>(v[r]);
+    keep_alive(pf);
+  }
+  flush(target);
 }
 
 // ---------------------------------------------------------------------------

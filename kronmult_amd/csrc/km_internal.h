@@ -37,6 +37,13 @@ bool lds_kernel(int d, int n, bool exact, bool fwd, KernelInfo *out);
 template<typename T, int D>
 bool lds_kernel_d(int n, bool exact, bool fwd, KernelInfo *out);
 
+#ifndef KM_USE_REG
+#define KM_USE_REG 1
+#endif
+// kron_reg instantiation for (d, n) if one was built (forward mode order)
+template<typename T>
+bool reg_kernel(int d, int n, bool exact, KernelInfo *out);
+
 // kron_slab instantiation for (d, n) if one was built
 template<typename T>
 bool slab_kernel(int d, int n, bool exact, KernelInfo *out);

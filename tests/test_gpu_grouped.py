@@ -1,5 +1,6 @@
-"""GPU parity of the grouped variable-size entry point (per-entry n, one
-persistent launch) against the CPU oracle, entry by entry."""
+"""GPU parity of the grouped variable-size entry point (per-entry n; per-size
+launches with atomic or owned outputs, or one persistent launch) against the
+CPU oracle, entry by entry."""
 from __future__ import annotations
 
 import numpy as np
@@ -32,7 +33,10 @@ def _oracle(d, ns, yg, mats, X, Y0):
     return Y
 
 
-def _gpu(km, d, ns, yg, mats, X, Y0, exact=False, runs=1, persistent=False):
+MODES = {"atomic": {}, "deterministic": {"deterministic": True}, "persistent": {"persistent": True}}
+
+
+def _gpu(km, d, ns, yg, mats, X, Y0, exact=False, runs=1, mode="atomic"):
     import torch
     dev = torch.device("cuda:0")
     batch = len(ns)
@@ -48,7 +52,7 @@ def _gpu(km, d, ns, yg, mats, X, Y0, exact=False, runs=1, persistent=False):
     for _ in range(runs):
         for n in tY:
             tY[n].copy_(torch.tensor(Y0[n], device=dev))
-        km.kronmult_grouped(plan, table, pX, pY, opt=km.options(exact=exact, persistent=persistent))
+        km.kronmult_grouped(plan, table, pX, pY, opt=km.options(exact=exact, **MODES[mode]))
         torch.cuda.synchronize()
         outs.append({n: t.cpu().numpy() for n, t in tY.items()})
     info = plan.info()
@@ -65,17 +69,32 @@ def km():
 
 @pytest.mark.parametrize("d,sizes", [(4, [2, 3, 4, 5, 6, 7, 8]), (3, [1, 2, 5, 8]), (2, [3, 8]),
                                      (5, [2, 3, 6]), (6, [2, 3, 4]), (1, [4, 7])])
-@pytest.mark.parametrize("persistent", [False, True])
-def test_grouped_vs_oracle(km, d, sizes, persistent):
+@pytest.mark.parametrize("mode", list(MODES))
+def test_grouped_vs_oracle(km, d, sizes, mode):
     ns, yg, mats, X, Y0 = _make(d * 10 + len(sizes), d, sizes, 300, 5)
     want = _oracle(d, ns, yg, mats, X, Y0)
-    outs, (nout, ngroups, nmax) = _gpu(km, d, ns, yg, mats, X, Y0, runs=2, persistent=persistent)
+    outs, (nout, ngroups, nmax) = _gpu(km, d, ns, yg, mats, X, Y0, runs=2, mode=mode)
     assert nmax == max(ns)
     assert nout == len({(n, g) for n, g in zip(ns, yg)})
     for n in want:
         assert rel_err(outs[0][n], want[n]) <= RTOL_FP64, n
-        # deterministic: owned outputs, fixed order
-        assert np.array_equal(outs[0][n], outs[1][n])
+        for r in outs[1:]:
+            assert rel_err(r[n], want[n]) <= RTOL_FP64, n
+        if mode != "atomic":
+            # owned outputs, fixed order: bitwise run to run
+            assert np.array_equal(outs[0][n], outs[1][n])
+
+
+def test_grouped_large_bins_vs_oracle(km):
+    """Bins larger than one resident wave of workgroups, so the even split
+    cuts output runs and the atomic flush of a shared run is exercised."""
+    d, sizes = 3, [2, 4]
+    ns, yg, mats, X, Y0 = _make(77, d, sizes, 6000, 7)
+    want = _oracle(d, ns, yg, mats, X, Y0)
+    for mode in MODES:
+        outs, _ = _gpu(km, d, ns, yg, mats, X, Y0, mode=mode)
+        for n in want:
+            assert rel_err(outs[0][n], want[n]) <= RTOL_FP64, (mode, n)
 
 
 def test_grouped_exact_single_entry_outputs_bitwise(km):
@@ -89,8 +108,8 @@ def test_grouped_exact_single_entry_outputs_bitwise(km):
     for b, n in enumerate(ns):
         Y0.setdefault(n, np.zeros((len(ns), n ** d)))
     want = _oracle(d, ns, yg, mats, X, Y0)
-    for persistent in (False, True):
-        outs, _ = _gpu(km, d, ns, yg, mats, X, Y0, exact=True, persistent=persistent)
+    for mode in MODES:
+        outs, _ = _gpu(km, d, ns, yg, mats, X, Y0, exact=True, mode=mode)
         for n in want:
             assert np.array_equal(outs[0][n], want[n])
 

@@ -55,7 +55,7 @@ def test_pbatched_overlap_vs_oracle(km, d, n):
     assert rel_err(got, want) <= RTOL_FP64, (d, n, rel_err(got, want))
 
 
-@pytest.mark.parametrize("d,n", [(1, 5), (2, 7), (3, 8), (4, 4), (4, 6), (5, 3), (5, 7), (6, 2), (6, 4), (6, 5), (6, 6),
+@pytest.mark.parametrize("d,n", [(1, 5), (2, 7), (3, 8), (4, 4), (4, 6), (4, 8), (5, 3), (5, 4), (5, 7), (6, 2), (6, 4), (6, 5), (6, 6),
                                  (5, 8), (6, 7), (6, 8)])
 @pytest.mark.parametrize("variant", ["backward", "forward"])
 def test_exact_mode_bitwise(km, d, n, variant):
