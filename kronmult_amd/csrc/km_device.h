@@ -41,11 +41,7 @@ namespace km
 #endif
 constexpr int kLdsUnroll = KM_LDS_UNROLL;
 constexpr int kSlabUnroll = KM_SLAB_UNROLL;
-// kron_reg loads the next entry's X into registers during the current one
-#ifndef KM_REG_PREFETCH_X
-#define KM_REG_PREFETCH_X 0
-#endif
-constexpr bool kRegPrefetchX = KM_REG_PREFETCH_X != 0;
+
 
 // ---------------------------------------------------------------------------
 // arguments
@@ -333,17 +329,40 @@ __device__ __forceinline__ void keep_alive(T v)
 // wavefront that owns the buffer.  LDS traffic of one wavefront is performed
 // in order; the workgroup-scope fence waits for it and keeps the compiler
 // from moving LDS accesses across the barrier.
+// s_waitcnt immediate for gfx9: lgkmcnt(0), vmcnt and expcnt left at their
+// maximum (not waited on)
+constexpr int kWaitLgkm0 = 0xC07F;
+
+// Workgroup barrier for data exchanged through LDS only.  It waits for this
+// wave's LDS (and scalar) accesses, then for every wave, but not for vector
+// memory: global loads in flight (the next entry's X prefetched into
+// registers, the slab DMA into the other LDS buffer) keep running across it.
+// __syncthreads() would wait for them (on gfx9 vmcnt counts loads as well as
+// stores).  The empty asm statements stop the compiler from moving memory
+// accesses across the barrier.  This is no fence for global memory; the
+// kernels that use it never pass data between threads through global memory.
+__device__ __forceinline__ void lds_barrier()
+{
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 template<bool WAVE>
 __device__ __forceinline__ void pass_sync()
 {
   if constexpr (WAVE)
   {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    // one wavefront: its LDS accesses complete in order once lgkmcnt is 0
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
     __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
   }
   else
   {
-    __syncthreads();
+    lds_barrier();
   }
 }
 
@@ -404,7 +423,7 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
   {
     const int b = entry_at(a, p);
     T *yb = y_of(a, b);
-    __syncthreads(); // buf free: previous entry fully consumed
+    lds_barrier(); // buf free: previous entry fully consumed
 #pragma unroll
     for (int r = 0; r < R; r++)
     {
@@ -435,7 +454,7 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
         acc[r] = 0;
       target = yb;
     }
-    __syncthreads();
+    lds_barrier();
     lds_stage<T, D, N, NT, EXACT, FWD, 0>(buf, a, b, tid);
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -594,22 +613,27 @@ __global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
     return;
 
   C v[RR];
-  C xr[RR];
   T acc[RR];
   T *target = nullptr;
+  // an owned output run (a plan) starts from Y itself when FMAs are used, in
+  // the accumulator's own layout (positions tid*RR + r), and ends with plain
+  // stores; EXACT keeps Y + (sum) and shared outputs need atomics, so those
+  // start from zero and leave through one coalesced LDS transpose
+  const bool preload = !EXACT && a.owned;
 
-  auto load_x = [&](const T *xp) {
-#pragma unroll
-    for (int r = 0; r < RR; r++)
-      xr[r] = static_cast<C>(gld(xp + r * NTH + tid)); // layout 0: coalesced
-  };
-  // acc (last layout) -> LDS -> coalesced positions tid + NTH*k -> Y
   auto flush = [&](T *dst) {
-    __syncthreads();
+    if (preload)
+    {
+#pragma unroll
+      for (int r = 0; r < RR; r++)
+        gst(dst + tid * RR + r, acc[r]);
+      return;
+    }
+    lds_barrier();
 #pragma unroll
     for (int r = 0; r < RR; r++)
       buf[reg_pad<RR>(reg_index<D, N, RM>(NG - 1, tid, r))] = acc[r];
-    __syncthreads();
+    lds_barrier();
     T out[RR];
 #pragma unroll
     for (int k = 0; k < RR; k++)
@@ -617,11 +641,15 @@ __global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
     flush_run<RR, NTH, S, RR>(dst, out, tid, a.owned);
   };
 
-  if constexpr (kRegPrefetchX)
-    load_x(x_of(a, entry_at(a, p0)));
   for (int p = p0; p < p1; ++p)
   {
     const int b = entry_at(a, p);
+    // X first (layout 0: coalesced), so that waiting for it does not also
+    // wait for the flush issued below
+    const T *xp = x_of(a, b);
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      v[r] = static_cast<C>(gld(xp + r * NTH + tid));
     T *yb = y_of(a, b);
     if (yb != target)
     {
@@ -629,33 +657,23 @@ __global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
         flush(target);
 #pragma unroll
       for (int r = 0; r < RR; r++)
-        acc[r] = 0;
+        acc[r] = preload ? gld(yb + tid * RR + r) : T(0);
       target = yb;
     }
-    if constexpr (!kRegPrefetchX)
-      load_x(x_of(a, b));
-#pragma unroll
-    for (int r = 0; r < RR; r++)
-      v[r] = xr[r];
     T pf = 0;
     if (p + 1 < p1)
-    {
-      const int bn = entry_at(a, p + 1);
-      if constexpr (kRegPrefetchX)
-        load_x(x_of(a, bn));
-      pf = prefetch_a<D, N>(a, bn, tid);
-    }
+      pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
 #pragma unroll
     for (int g = 0; g < NG; g++)
     {
       reg_group<T, D, N, RM, EXACT, 0>(v, a, b, g);
       if (g + 1 < NG)
       {
-        __syncthreads(); // previous readers of buf are done
+        lds_barrier(); // previous readers of buf are done
 #pragma unroll
         for (int r = 0; r < RR; r++)
           buf[reg_pad<RR>(reg_index<D, N, RM>(g, tid, r))] = static_cast<T>(v[r]);
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int r = 0; r < RR; r++)
           v[r] = static_cast<C>(buf[reg_pad<RR>(reg_index<D, N, RM>(g + 1, tid, r))]);
@@ -732,9 +750,47 @@ __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
     int lo = f - hi * STRIDE;
     fiber_apply<N, EXACT, T>(buf + hi * STRIDE * N + lo, STRIDE, am);
   }
-  __syncthreads();
+  lds_barrier();
   if constexpr (K + 2 < D)
     slab_stage<T, D, N, NT, EXACT, K + 1>(buf, a, b, tid);
+}
+
+// Output runs of the slab kernels.  A run whose output the workgroup owns
+// (a plan) starts from Y itself when the contraction uses FMAs: the loads are
+// issued at the start of the run and are not waited for until the first
+// mode-1 update, after the slab's mode passes, and the end of the run is a
+// plain store.  EXACT keeps the reference's rounding Y + (sum), so it starts
+// from zero and reads Y back at the end; shared outputs use atomics.
+template<int R, int NT, int S, typename T, typename C>
+__device__ __forceinline__ void run_init(C (&acc)[R], const T *src, int tid,
+                                         bool preload)
+{
+#pragma unroll
+  for (int r = 0; r < R; r++)
+  {
+    const int q = tid + NT * r;
+    acc[r] = (preload && (S % NT == 0 || q < S)) ? static_cast<C>(gld(src + q)) : C(0);
+  }
+}
+
+template<int R, int NT, int S, typename T, typename C>
+__device__ __forceinline__ void run_flush(T *dst, const C (&acc)[R], int tid,
+                                          int owned, bool preloaded)
+{
+  if (preloaded)
+  {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (S % NT == 0 || q < S)
+        gst(dst + q, static_cast<T>(acc[r]));
+    }
+  }
+  else
+  {
+    flush_run<R, NT, S, 2>(dst, acc, tid, owned);
+  }
 }
 
 template<typename T, int D, int N, int NT, bool EXACT>
@@ -754,6 +810,7 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
   using C = typename Acc<T>::type;
   C acc[N][RS];
   T *target = nullptr;
+  const bool preload = !EXACT && a.owned;
 
   // slab sequence: (entry p, j1) flattened as t = (p - p0)*N + j1
   const int nslab = (p1 - p0) * N;
@@ -770,6 +827,11 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
     T *cur = (t & 1) ? sb1 : sb0;
     T *nxt = (t & 1) ? sb0 : sb1;
 
+    // the current slab has landed for every wave; nxt is free again.  The
+    // wait also covers the stores of the previous run's flush, which were
+    // issued a whole slab earlier (below), so they drain behind compute.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
     if (j1 == 0)
     {
       T *yb = y_of(a, b);
@@ -779,20 +841,14 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
         {
 #pragma unroll
           for (int i1 = 0; i1 < N; i1++)
-            flush_run<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned);
+            run_flush<RS, NT, SL>(target + i1 * SL, acc[i1], tid, a.owned, preload);
         }
 #pragma unroll
         for (int i1 = 0; i1 < N; i1++)
-#pragma unroll
-          for (int r = 0; r < RS; r++)
-            acc[i1][r] = 0;
+          run_init<RS, NT, SL>(acc[i1], yb + i1 * SL, tid, preload);
         target = yb;
       }
     }
-
-    // the current slab has landed for every wave; nxt is free again
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
     if (t + 1 < nslab)
     {
       const int pn = p0 + (t + 1) / N;
@@ -823,7 +879,7 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
   }
 #pragma unroll
   for (int i1 = 0; i1 < N; i1++)
-    flush_run<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned);
+    run_flush<RS, NT, SL>(target + i1 * SL, acc[i1], tid, a.owned, preload);
 }
 
 // ---------------------------------------------------------------------------
@@ -1114,7 +1170,7 @@ __device__ __forceinline__ void trailing_stages(T *buf, const EntryArgs<T> &a,
   constexpr int F = SZ / N;
   constexpr int STRIDE = cpow(N, D - 1 - MODE);
   stage_apply<T, N, NT, EXACT, F, STRIDE>(buf, a_of(a, b, MODE, D), a.lda, tid);
-  __syncthreads();
+  lds_barrier();
   if constexpr (MODE > LAST)
     trailing_stages<T, D, N, NT, EXACT, LAST, MODE - 1>(buf, a, b, tid);
 }
@@ -1150,7 +1206,7 @@ __global__ __launch_bounds__(NT) void kron_sub(EntryArgs<T> a, int e0, int nsub)
     const int p = e0 + s / NN;
     const int js = s % NN;
     const int b = entry_at(a, p);
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int r = 0; r < R; r++)
     {
@@ -1170,7 +1226,7 @@ __global__ __launch_bounds__(NT) void kron_sub(EntryArgs<T> a, int e0, int nsub)
           xr[r] = gld(xp + q);
       }
     }
-    __syncthreads();
+    lds_barrier();
     trailing_stages<T, D, N, NT, EXACT, 2, D - 1>(buf, a, b, tid);
     T *wb = w_of(a, b, p - e0) + (long long)js * M;
 #pragma unroll

@@ -15,14 +15,10 @@ namespace km
     return true;                                                               \
   }
 
-#ifndef KM_SLAB_NT
-#define KM_SLAB_NT 512
-#endif
-
 template<>
 bool slab_kernel<double>(int d, int n, bool exact, KernelInfo *out)
 {
-  KM_SLAB_CASE(double, 6, 6, KM_SLAB_NT)
+  KM_SLAB_CASE(double, 6, 6, 512)
   KM_SLAB_CASE(double, 5, 8, 512)
   return false;
 }
