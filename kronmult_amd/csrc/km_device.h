@@ -292,6 +292,45 @@ __device__ __forceinline__ void flush_run(T *dst, const C (&acc)[R], int tid,
   }
 }
 
+// Output runs (consecutive entries with the same output, summed on chip).  A
+// run whose output the workgroup owns (a plan) starts from Y itself when the
+// contraction uses FMAs: the loads are issued at the start of the run and
+// are not waited for until the first update, after the entry's mode passes,
+// and the end of the run is a plain store.  EXACT keeps the reference's
+// rounding Y + (sum), so it starts from zero and reads Y back at the end;
+// shared outputs use atomics.
+template<int R, int NT, int S, typename T, typename C>
+__device__ __forceinline__ void run_init(C (&acc)[R], const T *src, int tid,
+                                         bool preload)
+{
+#pragma unroll
+  for (int r = 0; r < R; r++)
+  {
+    const int q = tid + NT * r;
+    acc[r] = (preload && (S % NT == 0 || q < S)) ? static_cast<C>(gld(src + q)) : C(0);
+  }
+}
+
+template<int R, int NT, int S, int CHUNK, typename T, typename C>
+__device__ __forceinline__ void run_flush(T *dst, const C (&acc)[R], int tid,
+                                          int owned, bool preloaded)
+{
+  if (preloaded)
+  {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (S % NT == 0 || q < S)
+        gst(dst + q, static_cast<T>(acc[r]));
+    }
+  }
+  else
+  {
+    flush_run<R, NT, S, CHUNK>(dst, acc, tid, owned);
+  }
+}
+
 // Pull the next entry's A matrices into L2 with one vector load per 128-byte
 // line; the value is consumed (keep_alive) only after the current entry, so
 // the later scalar loads of that entry hit L2 instead of HBM.
@@ -392,21 +431,18 @@ __device__ __forceinline__ void lds_stage(T *buf, const EntryArgs<T> &a, int b,
     lds_stage<T, D, N, NT, EXACT, FWD, K + 1, WAVE>(buf, a, b, tid);
 }
 
-template<typename T, int D, int N, int NT, bool EXACT, bool FWD>
-__global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
+template<typename T, int D, int N, int NT, bool EXACT, bool FWD, bool PRELOAD>
+__device__ __forceinline__ void lds_body(const EntryArgs<T> &a, T *buf, int p0,
+                                         int p1)
 {
   constexpr int S = cpow(N, D);
   constexpr int R = (S + NT - 1) / NT;
-  __shared__ T buf[S];
   const int tid = threadIdx.x;
-  int p0, p1;
-  wg_range(a, p0, p1);
-  if (p0 >= p1)
-    return;
 
   T acc[R];
   T xr[R];
   T *target = nullptr;
+  constexpr bool preload = PRELOAD;
 
   {
     const T *xp = x_of(a, entry_at(a, p0));
@@ -448,10 +484,8 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
     if (yb != target)
     {
       if (target)
-        flush_run<R, NT, S, 8>(target, acc, tid, a.owned);
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        acc[r] = 0;
+        run_flush<R, NT, S, 8>(target, acc, tid, a.owned, preload);
+      run_init<R, NT, S>(acc, yb, tid, preload);
       target = yb;
     }
     lds_barrier();
@@ -465,7 +499,23 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
     }
     keep_alive(pf);
   }
-  flush_run<R, NT, S, 8>(target, acc, tid, a.owned);
+  run_flush<R, NT, S, 8>(target, acc, tid, a.owned, preload);
+}
+
+template<typename T, int D, int N, int NT, bool EXACT, bool FWD>
+__global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
+{
+  __shared__ T buf[cpow(N, D)];
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+  // owned runs with FMAs start from Y (see run_init); one specialised body
+  // per case keeps each loop's code as tight as a kernel of its own
+  if (!EXACT && a.owned)
+    lds_body<T, D, N, NT, EXACT, FWD, !EXACT>(a, buf, p0, p1);
+  else
+    lds_body<T, D, N, NT, EXACT, FWD, false>(a, buf, p0, p1);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,8 +645,9 @@ __device__ __forceinline__ int reg_pad(int idx)
   return idx + idx / RR;
 }
 
-template<typename T, int D, int N, int RM, bool EXACT>
-__global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
+template<typename T, int D, int N, int RM, bool EXACT, bool PRELOAD>
+__device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
+                                         int p1)
 {
   using C = typename Acc<T>::type;
   constexpr int S = cpow(N, D);
@@ -605,12 +656,7 @@ __global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
   constexpr int NG = D / RM;
   static_assert(D % RM == 0, "mode groups must tile d");
   static_assert(NTH % 64 == 0, "whole wavefronts");
-  __shared__ T buf[S + S / RR];
   const int tid = threadIdx.x;
-  int p0, p1;
-  wg_range(a, p0, p1);
-  if (p0 >= p1)
-    return;
 
   C v[RR];
   T acc[RR];
@@ -619,7 +665,7 @@ __global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
   // the accumulator's own layout (positions tid*RR + r), and ends with plain
   // stores; EXACT keeps Y + (sum) and shared outputs need atomics, so those
   // start from zero and leave through one coalesced LDS transpose
-  const bool preload = !EXACT && a.owned;
+  constexpr bool preload = PRELOAD;
 
   auto flush = [&](T *dst) {
     if (preload)
@@ -685,6 +731,20 @@ __global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
     keep_alive(pf);
   }
   flush(target);
+}
+
+template<typename T, int D, int N, int RM, bool EXACT>
+__global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
+{
+  __shared__ T buf[cpow(N, D) + cpow(N, D - RM)];
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+  if (!EXACT && a.owned)
+    reg_body<T, D, N, RM, EXACT, !EXACT>(a, buf, p0, p1);
+  else
+    reg_body<T, D, N, RM, EXACT, false>(a, buf, p0, p1);
 }
 
 // ---------------------------------------------------------------------------
@@ -755,62 +815,18 @@ __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
     slab_stage<T, D, N, NT, EXACT, K + 1>(buf, a, b, tid);
 }
 
-// Output runs of the slab kernels.  A run whose output the workgroup owns
-// (a plan) starts from Y itself when the contraction uses FMAs: the loads are
-// issued at the start of the run and are not waited for until the first
-// mode-1 update, after the slab's mode passes, and the end of the run is a
-// plain store.  EXACT keeps the reference's rounding Y + (sum), so it starts
-// from zero and reads Y back at the end; shared outputs use atomics.
-template<int R, int NT, int S, typename T, typename C>
-__device__ __forceinline__ void run_init(C (&acc)[R], const T *src, int tid,
-                                         bool preload)
-{
-#pragma unroll
-  for (int r = 0; r < R; r++)
-  {
-    const int q = tid + NT * r;
-    acc[r] = (preload && (S % NT == 0 || q < S)) ? static_cast<C>(gld(src + q)) : C(0);
-  }
-}
-
-template<int R, int NT, int S, typename T, typename C>
-__device__ __forceinline__ void run_flush(T *dst, const C (&acc)[R], int tid,
-                                          int owned, bool preloaded)
-{
-  if (preloaded)
-  {
-#pragma unroll
-    for (int r = 0; r < R; r++)
-    {
-      const int q = tid + NT * r;
-      if (S % NT == 0 || q < S)
-        gst(dst + q, static_cast<T>(acc[r]));
-    }
-  }
-  else
-  {
-    flush_run<R, NT, S, 2>(dst, acc, tid, owned);
-  }
-}
-
-template<typename T, int D, int N, int NT, bool EXACT>
-__global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
+template<typename T, int D, int N, int NT, bool EXACT, bool PRELOAD>
+__device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
+                                          int p0, int p1)
 {
   constexpr int SL = cpow(N, D - 1);
   constexpr int RS = (SL + NT - 1) / NT;
-  static_assert((SL * sizeof(T)) % 16 == 0, "slab must be whole 16-byte chunks");
-  __shared__ __attribute__((aligned(16))) T sb0[SL];
-  __shared__ __attribute__((aligned(16))) T sb1[SL];
   const int tid = threadIdx.x;
-  int p0, p1;
-  wg_range(a, p0, p1);
-  if (p0 >= p1)
-    return;
 
   using C = typename Acc<T>::type;
   C acc[N][RS];
   T *target = nullptr;
-  const bool preload = !EXACT && a.owned;
+  constexpr bool preload = PRELOAD;
 
   // slab sequence: (entry p, j1) flattened as t = (p - p0)*N + j1
   const int nslab = (p1 - p0) * N;
@@ -841,7 +857,7 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
         {
 #pragma unroll
           for (int i1 = 0; i1 < N; i1++)
-            run_flush<RS, NT, SL>(target + i1 * SL, acc[i1], tid, a.owned, preload);
+            run_flush<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned, preload);
         }
 #pragma unroll
         for (int i1 = 0; i1 < N; i1++)
@@ -879,7 +895,24 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
   }
 #pragma unroll
   for (int i1 = 0; i1 < N; i1++)
-    run_flush<RS, NT, SL>(target + i1 * SL, acc[i1], tid, a.owned, preload);
+    run_flush<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned, preload);
+}
+
+template<typename T, int D, int N, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
+{
+  constexpr int SL = cpow(N, D - 1);
+  static_assert((SL * sizeof(T)) % 16 == 0, "slab must be whole 16-byte chunks");
+  __shared__ __attribute__((aligned(16))) T sb0[SL];
+  __shared__ __attribute__((aligned(16))) T sb1[SL];
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+  if (!EXACT && a.owned)
+    slab_body<T, D, N, NT, EXACT, !EXACT>(a, sb0, sb1, p0, p1);
+  else
+    slab_body<T, D, N, NT, EXACT, false>(a, sb0, sb1, p0, p1);
 }
 
 // ---------------------------------------------------------------------------
