@@ -250,6 +250,108 @@ km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
 /* Reference flop model (kron_flops, kroncommon.hpp:154-185) for one entry. */
 double km_kron_flops(int d, const int *m, const int *n);
 
+/* ---------------------------------------------------------------------------
+ * Pooled xbatched tables from index lists, built on the device: the step
+ * before km_kronmult_xbatched_* in a sparse-grid operator apply, where each
+ * entry b names one matrix per dimension from a pool, one input vector and
+ * one output vector.  No counterpart in the reference (its callers build the
+ * tables); the A table has the layout kronmultN_xbatched reads,
+ * Aarray[k + d*b] (kronmult_xbatched.hpp:67-69).
+ *   Apool, a_stride, a_count: HOST arrays of d entries: device base of pool
+ *     k, elements between consecutive matrices of pool k, matrices in pool k;
+ *   aidx: device int[d*batch], matrix of dimension k for entry b at [k + d*b];
+ *   X/x_stride/x_count, xidx: input pool (device), element stride between
+ *     vectors, vectors in the pool, device int[batch] index per entry;
+ *   Y/y_stride/y_count, yidx: the same for the outputs (indices may repeat);
+ *   Aarray (d*batch), pX, pY (batch): device outputs;
+ *   bad: optional device int, incremented once per index outside its pool
+ *     (that pointer is written as NULL).
+ * Asynchronous on `stream`; allocates nothing.
+ * ------------------------------------------------------------------------ */
+km_status km_xbatched_tables_f64(int d, int batchCount, const double *const *Apool,
+                                 const long long *a_stride, const int *a_count,
+                                 const int *aidx, double *X, long long x_stride,
+                                 int x_count, const int *xidx, double *Y,
+                                 long long y_stride, int y_count,
+                                 const int *yidx, const double **Aarray,
+                                 double **pX, double **pY, int *bad,
+                                 void *stream);
+km_status km_xbatched_tables_f32(int d, int batchCount, const float *const *Apool,
+                                 const long long *a_stride, const int *a_count,
+                                 const int *aidx, float *X, long long x_stride,
+                                 int x_count, const int *xidx, float *Y,
+                                 long long y_stride, int y_count,
+                                 const int *yidx, const float **Aarray,
+                                 float **pX, float **pY, int *bad,
+                                 void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched slender GEMM:  C_b = alpha*op(A_b)*op(B_b) + beta*C_b,  b < batchCount.
+ * Replaces the reference's kgemm_nn_batched / kgemm_nt_batched /
+ * kgemm_tt_batched (host wrappers kgemm_nn_batched.cpp:5-25,
+ * kgemm_nt_batched.cpp:5-25, kgemm_tt_batched.cpp:5-25; templates
+ * kgemm_nn_batched.hpp:7-63 and copies).
+ *   nn: A mm x kk, B kk x nn;  nt: A mm x kk, B nn x kk (C = A*B^T);
+ *   tt: A kk x mm, B nn x kk (C = A^T*B^T).  C is mm x nn.
+ *   Aarray/Barray/Carray and the leading-dimension arrays are device arrays
+ *   of batchCount entries (as in the reference); column-major storage.
+ *   beta == 1: C += alpha*(...) with atomic adds, so entries may share C;
+ *   beta == 0: C = alpha*(...), C not read;
+ *   otherwise C = beta*C + alpha*(...) (entries must not share C).
+ * Each element is a dot product over k ascending in double (float data is
+ * accumulated in double, kgemm_nn.hpp:214-221).  KM_FLAG_EXACT uses separate
+ * rounded multiplies and adds (the reference CPU build's rounding); the
+ * default fuses them.  Asynchronous on `stream`; allocates nothing.
+ * ------------------------------------------------------------------------ */
+km_status km_kgemm_nn_batched_f64(int mm, int nn, int kk, double alpha,
+                                  const double *const *Aarray,
+                                  const int *ldAarray,
+                                  const double *const *Barray,
+                                  const int *ldBarray, double beta,
+                                  double *const *Carray, const int *ldCarray,
+                                  int batchCount, void *stream,
+                                  const km_options *opt);
+km_status km_kgemm_nt_batched_f64(int mm, int nn, int kk, double alpha,
+                                  const double *const *Aarray,
+                                  const int *ldAarray,
+                                  const double *const *Barray,
+                                  const int *ldBarray, double beta,
+                                  double *const *Carray, const int *ldCarray,
+                                  int batchCount, void *stream,
+                                  const km_options *opt);
+km_status km_kgemm_tt_batched_f64(int mm, int nn, int kk, double alpha,
+                                  const double *const *Aarray,
+                                  const int *ldAarray,
+                                  const double *const *Barray,
+                                  const int *ldBarray, double beta,
+                                  double *const *Carray, const int *ldCarray,
+                                  int batchCount, void *stream,
+                                  const km_options *opt);
+km_status km_kgemm_nn_batched_f32(int mm, int nn, int kk, float alpha,
+                                  const float *const *Aarray,
+                                  const int *ldAarray,
+                                  const float *const *Barray,
+                                  const int *ldBarray, float beta,
+                                  float *const *Carray, const int *ldCarray,
+                                  int batchCount, void *stream,
+                                  const km_options *opt);
+km_status km_kgemm_nt_batched_f32(int mm, int nn, int kk, float alpha,
+                                  const float *const *Aarray,
+                                  const int *ldAarray,
+                                  const float *const *Barray,
+                                  const int *ldBarray, float beta,
+                                  float *const *Carray, const int *ldCarray,
+                                  int batchCount, void *stream,
+                                  const km_options *opt);
+km_status km_kgemm_tt_batched_f32(int mm, int nn, int kk, float alpha,
+                                  const float *const *Aarray,
+                                  const int *ldAarray,
+                                  const float *const *Barray,
+                                  const int *ldBarray, float beta,
+                                  float *const *Carray, const int *ldCarray,
+                                  int batchCount, void *stream,
+                                  const km_options *opt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,4 +95,25 @@ void kronmult6_vbatched(int const m1, int const n1, int const m2, int const n2, 
                         double *pY_[], double *W_, size_t const Wcapacity,
                         int const batchCount);
 
+// batched slender GEMMs (kgemm_nn_batched.cpp:5-10, kgemm_nt_batched.cpp:5-10,
+// kgemm_tt_batched.cpp:5-10)
+void kgemm_nn_batched(int const mm, int const nn, int const kk,
+                      double const alpha, double *const Aarray_[],
+                      int const ldAarray_[], double *const Barray_[],
+                      int const ldBarray_[], double const beta,
+                      double *const Carray_[], int const ldCarray_[],
+                      int const batchCount);
+void kgemm_nt_batched(int const mm, int const nn, int const kk,
+                      double const alpha, double *const Aarray_[],
+                      int const ldAarray_[], double *const Barray_[],
+                      int const ldBarray_[], double const beta,
+                      double *const Carray_[], int const ldCarray_[],
+                      int const batchCount);
+void kgemm_tt_batched(int const mm, int const nn, int const kk,
+                      double const alpha, double *const Aarray_[],
+                      int const ldAarray_[], double *const Barray_[],
+                      int const ldBarray_[], double const beta,
+                      double *const Carray_[], int const ldCarray_[],
+                      int const batchCount);
+
 #endif

@@ -61,6 +61,9 @@ C_ABI_SYMBOLS = (
     "km_query_kernel", "km_kron_flops",
     "km_group_plan_create", "km_group_plan_destroy", "km_group_plan_info",
     "km_kronmult_grouped_f64", "km_kronmult_grouped_f32",
+    "km_kgemm_nn_batched_f64", "km_kgemm_nt_batched_f64", "km_kgemm_tt_batched_f64",
+    "km_kgemm_nn_batched_f32", "km_kgemm_nt_batched_f32", "km_kgemm_tt_batched_f32",
+    "km_xbatched_tables_f64", "km_xbatched_tables_f32",
 )
 
 
@@ -149,6 +152,16 @@ def lib():
         L.km_group_plan_destroy.argtypes = [_P]
         L.km_group_plan_destroy.restype = _I
         L.km_group_plan_info.argtypes = [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]
+        for suf in ("f64", "f32"):
+            f = getattr(L, f"km_xbatched_tables_{suf}")
+            f.argtypes = [_I, _I, _P, _P, _P, _P, _P, C.c_longlong, _I, _P, _P, C.c_longlong, _I, _P,
+                          _P, _P, _P, _P, _P]
+            f.restype = _I
+        for op in ("nn", "nt", "tt"):
+            for suf, real in (("f64", C.c_double), ("f32", C.c_float)):
+                f = getattr(L, f"km_kgemm_{op}_batched_{suf}")
+                f.argtypes = [_I, _I, _I, real, _P, _P, _P, _P, real, _P, _P, _I, _P, C.POINTER(Options)]
+                f.restype = _I
         L.km_group_plan_info.restype = _I
         for suf in ("f64", "f32"):
             f = getattr(L, f"km_kronmult_grouped_{suf}")
@@ -262,6 +275,46 @@ def kronmult_vbatched(m: Sequence[int], n: Sequence[int], Aarray, pX, pY, W, Wca
     _check(f(d, C.cast(mm, C.c_void_p), C.cast(nn, C.c_void_p), _addr(Aarray), _addr(pX), _addr(pY),
              _addr(W), int(Wcapacity_bytes), batchCount, _stream(stream), _opt_ref(opt)),
            f"kronmult{d}_vbatched")
+
+
+def xbatched_tables(pools: Sequence, aidx, X, xidx, Y, yidx, bad=None, stream=None):
+    """Device-built pooled-xbatched tables (km_xbatched_tables_*).
+
+    ``pools[k]``: device tensor holding pool k's matrices, first index =
+    matrix; ``aidx``: device int32 ``(batch, d)``; ``X``/``Y``: device tensors
+    whose first index is the vector; ``xidx``/``yidx``: device int32
+    ``(batch,)``.  Returns ``(Aarray, pX, pY)`` as device int64 tensors of
+    pointers; ``bad`` (device int32, one element) counts out-of-pool indices."""
+    import torch
+    d = len(pools)
+    batch = int(aidx.shape[0])
+    dev = X.device
+    dt = _suffix(X)
+    P = (C.c_void_p * d)(*[p.data_ptr() for p in pools])
+    S = (C.c_longlong * d)(*[p.stride(0) for p in pools])
+    N = (C.c_int * d)(*[p.shape[0] for p in pools])
+    Aarray = torch.empty(d * batch, dtype=torch.int64, device=dev)
+    pX = torch.empty(batch, dtype=torch.int64, device=dev)
+    pY = torch.empty(batch, dtype=torch.int64, device=dev)
+    f = getattr(lib(), f"km_xbatched_tables_{dt}")
+    _check(f(d, batch, C.cast(P, C.c_void_p), C.cast(S, C.c_void_p), C.cast(N, C.c_void_p), _addr(aidx),
+             _addr(X), X.stride(0), X.shape[0], _addr(xidx), _addr(Y), Y.stride(0), Y.shape[0], _addr(yidx),
+             _addr(Aarray), _addr(pX), _addr(pY), _addr(bad), _stream(stream)), "xbatched_tables")
+    return Aarray, pX, pY
+
+
+def kgemm_batched(op: str, mm: int, nn: int, kk: int, alpha: float, Aarray, ldAarray, Barray, ldBarray,
+                  beta: float, Carray, ldCarray, batchCount: int, dtype="f64", stream=None,
+                  opt: Optional[Options] = None) -> None:
+    """kgemm_{nn,nt,tt}_batched: C_b = alpha*op(A_b)*op(B_b) + beta*C_b.
+    ``op`` is "nn", "nt" or "tt"; the six arrays are device arrays of
+    ``batchCount`` pointers / leading dimensions (int32)."""
+    if op not in ("nn", "nt", "tt"):
+        raise ValueError(f"op must be nn, nt or tt, not {op!r}")
+    f = getattr(lib(), f"km_kgemm_{op}_batched_{dtype}")
+    _check(f(mm, nn, kk, alpha, _addr(Aarray), _addr(ldAarray), _addr(Barray), _addr(ldBarray), beta,
+             _addr(Carray), _addr(ldCarray), batchCount, _stream(stream), _opt_ref(opt)),
+           f"kgemm_{op}_batched")
 
 
 # ---------------------------------------------------------------------------

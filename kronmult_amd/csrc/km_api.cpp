@@ -666,6 +666,93 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
   return KM_SUCCESS;
 }
 
+template<typename T>
+km_status kgemm(int op, int mm, int nn, int kk, T alpha, const T *const *A,
+                const int *ldA, const T *const *B, const int *ldB, T beta,
+                T *const *C, const int *ldC, int batch, void *stream,
+                const km_options *opt)
+{
+  if (mm < 0 || nn < 0 || kk < 0 || batch < 0)
+    return KM_ERROR_INVALID_VALUE;
+  if (mm == 0 || nn == 0 || batch == 0)
+    return KM_SUCCESS;
+  if (!A || !ldA || !B || !ldB || !C || !ldC)
+    return KM_ERROR_INVALID_VALUE;
+  km_options o;
+  km_options_init(&o);
+  if (opt)
+    o = *opt;
+  if (o.precision != KM_PRECISION_FP64 || o.plan)
+    return KM_ERROR_NOT_SUPPORTED;
+  const int cus = device_cus();
+  if (cus <= 0)
+  {
+    std::snprintf(g_last_error, sizeof(g_last_error), "no device / CU count");
+    return KM_ERROR_HIP;
+  }
+  km::KgemmArgs<T> g{};
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.lda = ldA;
+  g.ldb = ldB;
+  g.ldc = ldC;
+  g.mm = mm;
+  g.nn = nn;
+  g.kk = kk;
+  g.batch = batch;
+  g.alpha = alpha;
+  g.beta = beta;
+  KM_HIP(km::kgemm_launch<T>(op, g, (o.flags & KM_FLAG_EXACT) != 0, cus,
+                             (hipStream_t)stream));
+  return KM_SUCCESS;
+}
+
+template<typename T>
+km_status xbatched_tables(int d, int batch, const T *const *Apool,
+                          const long long *a_stride, const int *a_count,
+                          const int *aidx, T *X, long long x_stride, int x_count,
+                          const int *xidx, T *Y, long long y_stride, int y_count,
+                          const int *yidx, const T **Aarray, T **pX, T **pY,
+                          int *bad, void *stream)
+{
+  if (d < 1 || d > 6 || batch < 0)
+    return KM_ERROR_INVALID_VALUE;
+  if (batch == 0)
+    return KM_SUCCESS;
+  if (!Apool || !a_stride || !a_count || !aidx || !X || !xidx || !Y || !yidx ||
+      !Aarray || !pX || !pY || x_stride < 0 || y_stride < 0 || x_count < 0 ||
+      y_count < 0)
+    return KM_ERROR_INVALID_VALUE;
+  km::TableArgs<T> t{};
+  t.d = d;
+  t.batch = batch;
+  for (int k = 0; k < d; k++)
+  {
+    if (!Apool[k] || a_stride[k] < 0 || a_count[k] < 0)
+      return KM_ERROR_INVALID_VALUE;
+    t.pool[k] = Apool[k];
+    t.a_stride[k] = a_stride[k];
+    t.a_count[k] = a_count[k];
+  }
+  t.aidx = aidx;
+  t.xidx = xidx;
+  t.yidx = yidx;
+  t.X = X;
+  t.x_stride = x_stride;
+  t.x_count = x_count;
+  t.Y = Y;
+  t.y_stride = y_stride;
+  t.y_count = y_count;
+  t.Aarray = Aarray;
+  t.pX = pX;
+  t.pY = pY;
+  t.bad = bad;
+  const int cus = device_cus();
+  KM_HIP(km::tables_launch<T>(t, cus, (hipStream_t)stream));
+  return KM_SUCCESS;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1161,6 +1248,108 @@ extern "C"
                                     const km_options *opt)
   {
     return grouped<float>(plan, Aarray, pX, pY, stream, opt);
+  }
+
+  km_status km_kgemm_nn_batched_f64(int mm, int nn, int kk, double alpha,
+                                     const double *const *Aarray,
+                                     const int *ldAarray,
+                                     const double *const *Barray,
+                                     const int *ldBarray, double beta,
+                                     double *const *Carray, const int *ldCarray,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return kgemm<double>(0, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray,
+                      beta, Carray, ldCarray, batchCount, stream, opt);
+  }
+
+  km_status km_kgemm_nn_batched_f32(int mm, int nn, int kk, float alpha,
+                                     const float *const *Aarray,
+                                     const int *ldAarray,
+                                     const float *const *Barray,
+                                     const int *ldBarray, float beta,
+                                     float *const *Carray, const int *ldCarray,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return kgemm<float>(0, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray,
+                      beta, Carray, ldCarray, batchCount, stream, opt);
+  }
+
+  km_status km_kgemm_nt_batched_f64(int mm, int nn, int kk, double alpha,
+                                     const double *const *Aarray,
+                                     const int *ldAarray,
+                                     const double *const *Barray,
+                                     const int *ldBarray, double beta,
+                                     double *const *Carray, const int *ldCarray,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return kgemm<double>(1, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray,
+                      beta, Carray, ldCarray, batchCount, stream, opt);
+  }
+
+  km_status km_kgemm_nt_batched_f32(int mm, int nn, int kk, float alpha,
+                                     const float *const *Aarray,
+                                     const int *ldAarray,
+                                     const float *const *Barray,
+                                     const int *ldBarray, float beta,
+                                     float *const *Carray, const int *ldCarray,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return kgemm<float>(1, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray,
+                      beta, Carray, ldCarray, batchCount, stream, opt);
+  }
+
+  km_status km_kgemm_tt_batched_f64(int mm, int nn, int kk, double alpha,
+                                     const double *const *Aarray,
+                                     const int *ldAarray,
+                                     const double *const *Barray,
+                                     const int *ldBarray, double beta,
+                                     double *const *Carray, const int *ldCarray,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return kgemm<double>(2, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray,
+                      beta, Carray, ldCarray, batchCount, stream, opt);
+  }
+
+  km_status km_kgemm_tt_batched_f32(int mm, int nn, int kk, float alpha,
+                                     const float *const *Aarray,
+                                     const int *ldAarray,
+                                     const float *const *Barray,
+                                     const int *ldBarray, float beta,
+                                     float *const *Carray, const int *ldCarray,
+                                     int batchCount, void *stream,
+                                     const km_options *opt)
+  {
+    return kgemm<float>(2, mm, nn, kk, alpha, Aarray, ldAarray, Barray, ldBarray,
+                      beta, Carray, ldCarray, batchCount, stream, opt);
+  }
+
+  km_status km_xbatched_tables_f64(
+      int d, int batchCount, const double *const *Apool, const long long *a_stride,
+      const int *a_count, const int *aidx, double *X, long long x_stride,
+      int x_count, const int *xidx, double *Y, long long y_stride, int y_count,
+      const int *yidx, const double **Aarray, double **pX, double **pY, int *bad,
+      void *stream)
+  {
+    return xbatched_tables<double>(d, batchCount, Apool, a_stride, a_count, aidx, X,
+                               x_stride, x_count, xidx, Y, y_stride, y_count,
+                               yidx, Aarray, pX, pY, bad, stream);
+  }
+
+  km_status km_xbatched_tables_f32(
+      int d, int batchCount, const float *const *Apool, const long long *a_stride,
+      const int *a_count, const int *aidx, float *X, long long x_stride,
+      int x_count, const int *xidx, float *Y, long long y_stride, int y_count,
+      const int *yidx, const float **Aarray, float **pX, float **pY, int *bad,
+      void *stream)
+  {
+    return xbatched_tables<float>(d, batchCount, Apool, a_stride, a_count, aidx, X,
+                               x_stride, x_count, xidx, Y, y_stride, y_count,
+                               yidx, Aarray, pX, pY, bad, stream);
   }
 
   double km_kron_flops(int d, const int *m, const int *n)

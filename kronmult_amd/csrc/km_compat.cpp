@@ -258,3 +258,42 @@ void kronmult6_vbatched(int const m1, int const n1, int const m2, int const n2, 
                                  batchCount, nullptr, nullptr),
         "kronmult6_vbatched");
 }
+
+void kgemm_nn_batched(int const mm, int const nn, int const kk,
+                      double const alpha, double *const Aarray_[],
+                      int const ldAarray_[], double *const Barray_[],
+                      int const ldBarray_[], double const beta,
+                      double *const Carray_[], int const ldCarray_[],
+                      int const batchCount)
+{
+  check(km_kgemm_nn_batched_f64(mm, nn, kk, alpha, Aarray_, ldAarray_, Barray_,
+                                ldBarray_, beta, Carray_, ldCarray_, batchCount,
+                                nullptr, nullptr),
+        "kgemm_nn_batched");
+}
+
+void kgemm_nt_batched(int const mm, int const nn, int const kk,
+                      double const alpha, double *const Aarray_[],
+                      int const ldAarray_[], double *const Barray_[],
+                      int const ldBarray_[], double const beta,
+                      double *const Carray_[], int const ldCarray_[],
+                      int const batchCount)
+{
+  check(km_kgemm_nt_batched_f64(mm, nn, kk, alpha, Aarray_, ldAarray_, Barray_,
+                                ldBarray_, beta, Carray_, ldCarray_, batchCount,
+                                nullptr, nullptr),
+        "kgemm_nt_batched");
+}
+
+void kgemm_tt_batched(int const mm, int const nn, int const kk,
+                      double const alpha, double *const Aarray_[],
+                      int const ldAarray_[], double *const Barray_[],
+                      int const ldBarray_[], double const beta,
+                      double *const Carray_[], int const ldCarray_[],
+                      int const batchCount)
+{
+  check(km_kgemm_tt_batched_f64(mm, nn, kk, alpha, Aarray_, ldAarray_, Barray_,
+                                ldBarray_, beta, Carray_, ldCarray_, batchCount,
+                                nullptr, nullptr),
+        "kgemm_tt_batched");
+}

@@ -37,6 +37,50 @@ bool lds_kernel(int d, int n, bool exact, bool fwd, KernelInfo *out);
 template<typename T, int D>
 bool lds_kernel_d(int n, bool exact, bool fwd, KernelInfo *out);
 
+// batched slender GEMM (km_kgemm.hip); op 0 = nn, 1 = nt, 2 = tt
+template<typename T>
+struct KgemmArgs
+{
+  const T *const *A;
+  const T *const *B;
+  T *const *C;
+  const int *lda;
+  const int *ldb;
+  const int *ldc;
+  int mm, nn, kk, batch;
+  int tiles; // row tiles per entry (set by the launcher)
+  typename Acc<T>::type alpha, beta;
+};
+template<typename T>
+hipError_t kgemm_launch(int op, KgemmArgs<T> &g, bool exact, int cus,
+                        hipStream_t stream);
+
+// pooled xbatched tables from index lists (km_tables.hip)
+template<typename T>
+struct TableArgs
+{
+  int d;
+  long long batch;
+  const T *pool[6];
+  long long a_stride[6];
+  int a_count[6];
+  const int *aidx;
+  const int *xidx;
+  const int *yidx;
+  T *X;
+  long long x_stride;
+  int x_count;
+  T *Y;
+  long long y_stride;
+  int y_count;
+  const T **Aarray;
+  T **pX;
+  T **pY;
+  int *bad;
+};
+template<typename T>
+hipError_t tables_launch(TableArgs<T> &t, int cus, hipStream_t stream);
+
 #ifndef KM_USE_REG
 #define KM_USE_REG 1
 #endif

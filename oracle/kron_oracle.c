@@ -24,7 +24,9 @@
  *     sub-batches);
  *   - the brute-force Kronecker sum of the reference tests:
  *     test_kronmult6_xbatched.cpp:397-500;
- *   - the flop model kron_flops: kroncommon.hpp:154-185.
+ *   - the flop model kron_flops: kroncommon.hpp:154-185;
+ *   - the standalone batched slender GEMMs kgemm_{nn,nt,tt}_batched
+ *     (kgemm_nn_batched.hpp:7-63 and copies), with their alpha/beta rules.
  *
  * Float entry points follow the reference's float templates: products and
  * sums in double, every stage result rounded to float when stored
@@ -428,6 +430,66 @@ double oracle_kron_flops(int nd, const int *m, const int *n)
 
 DEFINE_ORACLE(double, double, f64)
 DEFINE_ORACLE(float, double, f32)
+
+/*
+ * Batched slender GEMM, the reference's kgemm_{nn,nt,tt}_batched
+ * (kgemm_nn_batched.hpp:7-63, kgemm_nt_batched.hpp:7-62,
+ * kgemm_tt_batched.hpp:7-62; host wrappers kgemm_nn_batched.cpp:5-25 and the
+ * nt/tt copies).  Entries run in batch order (the CPU build's loop,
+ * kgemm_nn_batched.hpp:53-62); each element of entry b is
+ *   c = sum_k op(A)(i,k) * op(B)(k,j)    k ascending from 0, in ACC
+ * (kgemm_nn.hpp:138-177, kgemm_nt.hpp:138-177, kgemm_tt.hpp:136-175), then
+ *   beta == 1:  C += (T)(alpha*c)          atomic add (kgemm_nn.hpp:182-186)
+ *   beta == 0:  C  = (T)(alpha*c)
+ *   otherwise:  nn:    C = beta*C + (T)(alpha*c)  (alpha*c rounded to T first,
+ *                                                  kgemm_nn.hpp:180, 191)
+ *               nt/tt: C = beta*C + alpha*c       (kgemm_nt.hpp:192-194,
+ *                                                  kgemm_tt.hpp:190-192)
+ * with alpha and beta taken to ACC (kgemm_nn.hpp:32-33).
+ * op = 0: nn, A mm x kk, B kk x nn;  op = 1: nt, B nn x kk;
+ * op = 2: tt, A kk x mm, B nn x kk.  All column-major with per-entry leading
+ * dimensions; C is mm x nn.
+ */
+#define DEFINE_KGEMM_ORACLE(T, ACC, SUF)                                       \
+  void oracle_kgemm_batched_##SUF(int op, int mm, int nn, int kk, T alpha,     \
+                                  const T *const *A, const int *ldA,           \
+                                  const T *const *B, const int *ldB, T beta,   \
+                                  T *const *C, const int *ldC, int batch)      \
+  {                                                                            \
+    const ACC al = alpha, be = beta;                                           \
+    for (int b = 0; b < batch; b++)                                            \
+    {                                                                          \
+      const T *a = A[b], *bm = B[b];                                           \
+      T *cm = C[b];                                                            \
+      const size_t la = ldA[b], lb = ldB[b], lc = ldC[b];                      \
+      for (int j = 0; j < nn; j++)                                             \
+        for (int i = 0; i < mm; i++)                                           \
+        {                                                                      \
+          ACC c = 0;                                                           \
+          for (int k = 0; k < kk; k++)                                         \
+          {                                                                    \
+            ACC x = (op == 2) ? a[k + i * la] : a[i + k * la];                 \
+            ACC y = (op == 0) ? bm[k + j * lb] : bm[j + k * lb];               \
+            c += x * y;                                                        \
+          }                                                                    \
+          T *cp = &cm[i + j * lc];                                             \
+          if (be == 1)                                                         \
+            *cp += (T)(al * c);                                                \
+          else if (be == 0)                                                    \
+            *cp = (T)(al * c);                                                 \
+          else if (op == 0)                                                    \
+          {                                                                    \
+            const T acij = (T)(al * c);                                        \
+            *cp = (T)(be * (ACC)(*cp) + (ACC)acij);                            \
+          }                                                                    \
+          else                                                                 \
+            *cp = (T)(be * (ACC)(*cp) + al * c);                               \
+        }                                                                      \
+    }                                                                          \
+  }
+
+DEFINE_KGEMM_ORACLE(double, double, f64)
+DEFINE_KGEMM_ORACLE(float, double, f32)
 
 int oracle_num_threads(void)
 {

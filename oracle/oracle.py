@@ -56,6 +56,12 @@ def _declare(lib, prefix):
     return lib
 
 
+def _declare_kgemm(lib, prefix):
+    for suf, real in (("f64", C.c_double), ("f32", C.c_float)):
+        f = getattr(lib, f"{prefix}_kgemm_batched_{suf}")
+        f.argtypes = [_I, _I, _I, _I, real, _P, _P, _P, _P, real, _P, _P, _I]
+
+
 def lib():
     """The restatement library (raises if it has not been built)."""
     global _lib
@@ -73,6 +79,7 @@ def lib():
         L.oracle_alg993_uses_forward.argtypes = [_I, _P, _P]
         L.oracle_num_threads.restype = _I
         L.oracle_set_num_threads.argtypes = [_I]
+        _declare_kgemm(L, "oracle")
         _lib = L
     return _lib
 
@@ -93,6 +100,7 @@ def ref_lib(variant: str = "recursive"):
         L.ref_kron_flops.argtypes = [_I, _P, _P]
         L.ref_kron_flops.restype = C.c_double
         L.ref_variant.restype = _I
+        _declare_kgemm(L, "ref")
         _ref[variant] = L
     return _ref[variant]
 
@@ -300,3 +308,56 @@ def entry(mats: Sequence[np.ndarray], x: np.ndarray, variant: int = RECURSIVE) -
     y = np.zeros(int(np.prod(m)), dtype=dt)
     getattr(lib(), f"oracle_entry_{_suffix(dt)}")(d, _ptr(m), _ptr(n), _ptr(Aptr), _ptr(ld), variant, _ptr(xx), _ptr(y), _ptr(w))
     return y
+
+
+# ---------------------------------------------------------------------------
+# batched slender GEMM (kgemm_{nn,nt,tt}_batched)
+# ---------------------------------------------------------------------------
+KGEMM_NN, KGEMM_NT, KGEMM_TT = 0, 1, 2
+
+
+def colmajor(M: np.ndarray, ld: Optional[int] = None, dtype=None) -> np.ndarray:
+    """Column-major storage of matrix M with leading dimension ``ld`` >= rows:
+    a Fortran-ordered ``(ld, cols)`` array whose first ``rows`` rows hold M."""
+    M = np.asarray(M)
+    rows, cols = M.shape
+    ld = rows if ld is None else int(ld)
+    assert ld >= max(rows, 1)
+    buf = np.zeros((ld, cols), dtype=dtype or M.dtype, order="F")
+    buf[:rows, :] = M
+    return buf
+
+
+def kgemm_batched(op: int, mm: int, nn: int, kk: int, alpha: float, As: Sequence[np.ndarray],
+                  Bs: Sequence[np.ndarray], beta: float, Cs: Sequence[np.ndarray],
+                  use_ref: bool = False) -> list:
+    """C_b = alpha*op(A_b)*op(B_b) + beta*C_b for every entry b in order.
+
+    ``As``, ``Bs``, ``Cs`` hold one column-major buffer per entry (see
+    :func:`colmajor`; leading dimension = first extent).  Entries may share
+    a C buffer (the same array object), as callers of the reference may.
+    Returns the updated C buffers per entry (copies; shared inputs stay
+    shared).  op: KGEMM_NN (A mm x kk, B kk x nn), KGEMM_NT (B nn x kk),
+    KGEMM_TT (A kk x mm, B nn x kk)."""
+    batch = len(As)
+    dt = np.dtype(As[0].dtype) if batch else np.dtype(np.float64)
+    suf = _suffix(dt)
+    work = {}
+    outs = []
+    for c in Cs:
+        if id(c) not in work:
+            work[id(c)] = np.array(c, dtype=dt, order="F", copy=True)
+        outs.append(work[id(c)])
+    Ak = [np.asfortranarray(a, dtype=dt) for a in As]
+    Bk = [np.asfortranarray(b, dtype=dt) for b in Bs]
+    pa = np.array([_ptr(a) for a in Ak], dtype=np.uintp)
+    pb = np.array([_ptr(b) for b in Bk], dtype=np.uintp)
+    pc = np.array([_ptr(c) for c in outs], dtype=np.uintp)
+    la = _ivec([a.shape[0] for a in Ak])
+    lb = _ivec([b.shape[0] for b in Bk])
+    lc = _ivec([c.shape[0] for c in outs])
+    L = ref_lib() if use_ref else lib()
+    f = getattr(L, f"{'ref' if use_ref else 'oracle'}_kgemm_batched_{suf}")
+    f(int(op), int(mm), int(nn), int(kk), float(alpha), _ptr(pa), _ptr(la), _ptr(pb), _ptr(lb),
+      float(beta), _ptr(pc), _ptr(lc), int(batch))
+    return outs

@@ -14,6 +14,7 @@
 //   kronmult{1..6}_pbatched<T>  kronmult6_pbatched.hpp:12-67
 //   kronmult{1..6}_xbatched<T>  kronmult6_xbatched.hpp:10-31 -> kronmult_xbatched.hpp:17-162
 //   kronmult{1..6}_vbatched<T>  kronmult6_vbatched.hpp:10-34 -> kronmult_vbatched.hpp:17-228
+//   kgemm_{nn,nt,tt}_batched<T> kgemm_nn_batched.hpp:7-63 (and the nt/tt copies)
 #include <cstddef>
 #include <cstdio>
 
@@ -37,6 +38,10 @@
 #include "kronmult4_xbatched.hpp"
 #include "kronmult5_xbatched.hpp"
 #include "kronmult6_xbatched.hpp"
+
+#include "kgemm_nn_batched.hpp"
+#include "kgemm_nt_batched.hpp"
+#include "kgemm_tt_batched.hpp"
 
 #include "kronmult1_vbatched.hpp"
 #include "kronmult2_vbatched.hpp"
@@ -209,4 +214,41 @@ extern "C"
   {
     return kron_flops(nd, m, n);
   }
+}
+
+namespace
+{
+template<typename T>
+int kgemm(int op, int mm, int nn, int kk, T alpha, T const *const *A,
+          int const *ldA, T const *const *B, int const *ldB, T beta,
+          T *const *C, int const *ldC, int batch)
+{
+  T *const *a = const_cast<T *const *>(A);
+  T *const *b = const_cast<T *const *>(B);
+  switch (op)
+  {
+  case 0: kgemm_nn_batched<T>(mm, nn, kk, alpha, a, ldA, b, ldB, beta, C, ldC, batch); return 0;
+  case 1: kgemm_nt_batched<T>(mm, nn, kk, alpha, a, ldA, b, ldB, beta, C, ldC, batch); return 0;
+  case 2: kgemm_tt_batched<T>(mm, nn, kk, alpha, a, ldA, b, ldB, beta, C, ldC, batch); return 0;
+  }
+  return -1;
+}
+} // namespace
+
+extern "C" int ref_kgemm_batched_f64(int op, int mm, int nn, int kk, double alpha,
+                                     double const *const *A, int const *ldA,
+                                     double const *const *B, int const *ldB,
+                                     double beta, double *const *C,
+                                     int const *ldC, int batch)
+{
+  return kgemm<double>(op, mm, nn, kk, alpha, A, ldA, B, ldB, beta, C, ldC, batch);
+}
+
+extern "C" int ref_kgemm_batched_f32(int op, int mm, int nn, int kk, float alpha,
+                                     float const *const *A, int const *ldA,
+                                     float const *const *B, int const *ldB,
+                                     float beta, float *const *C,
+                                     int const *ldC, int batch)
+{
+  return kgemm<float>(op, mm, nn, kk, alpha, A, ldA, B, ldB, beta, C, ldC, batch);
 }

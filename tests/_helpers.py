@@ -162,3 +162,84 @@ def gpu_vbatched(m, n, mats, X, Y0, yidx, Wcap=None, opt=None):
     torch.cuda.synchronize()
     assert np.array_equal(tX.cpu().numpy(), X), "X was modified"
     return tY.cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+# batched slender GEMM (kgemm_{nn,nt,tt}_batched)
+# ---------------------------------------------------------------------------
+KGEMM_OP = {"nn": 0, "nt": 1, "tt": 2}
+# the reference tests' acceptance bounds on max |error| (FP64):
+# test_kgemm_nn_batched.cpp:570, test_kgemm_nt_batched.cpp:584,
+# test_kgemm_tt_batched.cpp:588
+KGEMM_REF_TOL = {"nn": 1e-9, "nt": 1e-6, "tt": 1e-6}
+
+
+def kgemm_shapes(op: str, mm: int, nn: int, kk: int):
+    """(rows, cols) of A and B for op (C is mm x nn)."""
+    a = (kk, mm) if op == "tt" else (mm, kk)
+    b = (kk, nn) if op == "nn" else (nn, kk)
+    return a, b
+
+
+def kgemm_inputs(op: str, mm: int, nn: int, kk: int, batch: int, kind: str = "random", seed: int = 0,
+                 dtype=np.float64, pad: int = 0, shared_c: bool = False):
+    """Per-entry column-major buffers (oracle.colmajor) As, Bs, Cs.
+
+    kind: "random" U[-1,1); "reference" the reference tests' closed form
+    A(i,j) = 1 + i + j*nrowA + b, B(i,j) = 1/(1 + i + j*nrowB + b), C = 1
+    (1-based i, j; test_kgemm_nt_batched.cpp:174-197); "integer" values in
+    {-2..2} (every sum exact).  pad adds to every leading dimension.
+    shared_c: all entries update one C buffer (the same object)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    (ra, ca), (rb, cb) = kgemm_shapes(op, mm, nn, kk)
+
+    def make(rows, cols, b, which):
+        if kind == "reference":
+            i = np.arange(1, rows + 1)[:, None]
+            j = np.arange(1, cols + 1)[None, :]
+            if which == "A":
+                return 1.0 + i + j * rows + b
+            if which == "B":
+                return 1.0 / (1.0 + i + j * rows + b)
+            return np.ones((rows, cols))
+        if kind == "integer":
+            return rng.integers(-2, 3, (rows, cols)).astype(np.float64)
+        return rng.uniform(-1, 1, (rows, cols))
+
+    As = [O.colmajor(make(ra, ca, b, "A"), ld=ra + pad, dtype=dtype) for b in range(batch)]
+    Bs = [O.colmajor(make(rb, cb, b, "B"), ld=rb + pad, dtype=dtype) for b in range(batch)]
+    if shared_c:
+        c = O.colmajor(make(mm, nn, 0, "C"), ld=mm + pad, dtype=dtype)
+        Cs = [c] * batch
+    else:
+        Cs = [O.colmajor(make(mm, nn, b, "C"), ld=mm + pad, dtype=dtype) for b in range(batch)]
+    return As, Bs, Cs
+
+
+def gpu_kgemm(op: str, mm: int, nn: int, kk: int, alpha: float, As, Bs, beta: float, Cs, opt=None):
+    """Run kgemm_{op}_batched on the GPU through the C ABI; returns the C
+    buffers per entry (shared inputs stay shared)."""
+    import kronmult_amd as km
+    torch = _torch()
+    dev = torch.device("cuda:0")
+    dt = "f64" if As[0].dtype == np.float64 else "f32"
+    tA = [torch.tensor(np.ascontiguousarray(a.T), device=dev) for a in As]
+    tB = [torch.tensor(np.ascontiguousarray(b.T), device=dev) for b in Bs]
+    uniq = {}
+    for c in Cs:
+        if id(c) not in uniq:
+            uniq[id(c)] = (c, torch.tensor(np.ascontiguousarray(c.T), device=dev))
+    tC = [uniq[id(c)][1] for c in Cs]
+    i64 = dict(dtype=torch.int64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    pA = torch.tensor([t.data_ptr() for t in tA], **i64)
+    pB = torch.tensor([t.data_ptr() for t in tB], **i64)
+    pC = torch.tensor([t.data_ptr() for t in tC], **i64)
+    lA = torch.tensor([a.shape[0] for a in As], **i32)
+    lB = torch.tensor([b.shape[0] for b in Bs], **i32)
+    lC = torch.tensor([c.shape[0] for c in Cs], **i32)
+    km.kgemm_batched(op, mm, nn, kk, alpha, pA, lA, pB, lB, beta, pC, lC, len(As), dtype=dt, opt=opt)
+    torch.cuda.synchronize()
+    out = {k: np.asfortranarray(t.cpu().numpy().T) for k, (c, t) in uniq.items()}
+    return [out[id(c)] for c in Cs]

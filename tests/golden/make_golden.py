@@ -160,13 +160,50 @@ def flops_cases():
             "flops": np.array(vals)}
 
 
+def kgemm_cases():
+    """kgemm_{nn,nt,tt}_batched: shapes from the reference tests' grid
+    (test_kgemm_nt_batched.cpp:490-500) and slender ones, padded leading
+    dimensions, beta in {0, 1, 1.2} (1: every entry adds into one shared C),
+    double and float."""
+    from _helpers import kgemm_inputs
+    out = {}
+    shapes = [(1, 1, 1), (8, 8, 8), (15, 1, 22), (29, 12, 8), (64, 8, 8), (1, 13, 22)]
+    case = 0
+    for op in ("nn", "nt", "tt"):
+        for dt in (np.float64, np.float32):
+            for si, (mm, nn, kk) in enumerate(shapes):
+                for beta in (0.0, 1.0, 1.2):
+                    kind = "reference" if si % 2 == 0 else "random"
+                    shared = beta == 1.0
+                    As, Bs, Cs = kgemm_inputs(op, mm, nn, kk, 2, kind=kind, seed=case, dtype=dt, pad=si % 3,
+                                              shared_c=shared)
+                    C = O.kgemm_batched(O.KGEMM_NN + ("nn", "nt", "tt").index(op), mm, nn, kk, 1.3, As, Bs,
+                                        beta, Cs, use_ref=True)
+                    key = f"c{case}"
+                    out[key + "_meta"] = np.array([("nn", "nt", "tt").index(op), mm, nn, kk, int(shared)])
+                    out[key + "_alpha"] = np.array([1.3], dtype=dt)
+                    out[key + "_beta"] = np.array([beta], dtype=dt)
+                    for name, bufs in (("A", As), ("B", Bs)):
+                        for b, buf in enumerate(bufs):
+                            out[f"{key}_{name}{b}"] = buf
+                    cs = Cs[:1] if shared else Cs
+                    outs = C[:1] if shared else C
+                    for b, (c0, c1) in enumerate(zip(cs, outs)):
+                        out[f"{key}_C0{b}"] = c0
+                        out[f"{key}_C{b}"] = c1
+                    case += 1
+    out["ncases"] = np.array([case])
+    return out
+
+
 def main():
     for b in BUILDS:
         if not O.ref_available(b):
             raise SystemExit(f"reference CPU build {b} missing: run `make -C oracle ref`")
     O.set_num_threads(1)
     for name, fn in (("pbatched", pbatched_cases), ("batched", batched_cases), ("xbatched", xbatched_cases),
-                     ("vbatched", vbatched_cases), ("pbatched_f32", float_cases), ("kron_flops", flops_cases)):
+                     ("vbatched", vbatched_cases), ("pbatched_f32", float_cases), ("kron_flops", flops_cases),
+                     ("kgemm", kgemm_cases)):
         data = fn()
         path = os.path.join(HERE, f"ref_{name}.npz")
         np.savez_compressed(path, **data)
