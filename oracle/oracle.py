@@ -136,6 +136,16 @@ def _rowptrs(arr: np.ndarray, idx: Sequence[int]) -> np.ndarray:
     return (base + np.asarray(idx, dtype=np.int64) * stride).astype(np.uintp)
 
 
+def _check_index(idx, batch: int, rows: int, name: str) -> None:
+    """Index vectors become raw pointers in C: reject a wrong shape or an
+    out-of-range entry here instead of reading out of bounds."""
+    idx = np.asarray(idx)
+    if idx.shape != (batch,) or not np.issubdtype(idx.dtype, np.integer):
+        raise ValueError(f"{name} must be {batch} integers, got shape {idx.shape} dtype {idx.dtype}")
+    if batch and (idx.min() < 0 or idx.max() >= rows):
+        raise ValueError(f"{name} out of range [0, {rows})")
+
+
 def _ivec(v: Sequence[int]) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(v, dtype=np.int32))
 
@@ -181,6 +191,8 @@ def pbatched(A: np.ndarray, X: np.ndarray, Y: np.ndarray, yidx: Sequence[int],
     batch, d, n, _ = A.shape
     A = np.ascontiguousarray(A)
     xidx = np.arange(batch) if xidx is None else np.asarray(xidx)
+    _check_index(xidx, batch, X.shape[0], "xidx")
+    _check_index(yidx, batch, Y.shape[0], "yidx")
     Xw = np.ascontiguousarray(X[xidx]).copy()
     Yo = np.array(Y, copy=True, order="C")
     W = np.zeros_like(Xw)
@@ -214,6 +226,10 @@ def xbatched(Apool: Sequence[np.ndarray], aidx: np.ndarray, X: np.ndarray, xidx:
     n = Apool[0].shape[1]
     lda = Apool[0].shape[2]
     dt = Apool[0].dtype
+    for k in range(d):
+        _check_index(aidx[:, k], batch, Apool[k].shape[0], f"aidx[:, {k}]")
+    _check_index(xidx, batch, X.shape[0], "xidx")
+    _check_index(yidx, batch, Y.shape[0], "yidx")
     pools = [np.ascontiguousarray(p) for p in Apool]
     Aptr = np.empty((batch, d), dtype=np.uintp)
     for k in range(d):
@@ -245,6 +261,9 @@ def vbatched(m: Sequence[int], n: Sequence[int], A: Sequence[Sequence[np.ndarray
     d = len(m)
     batch = len(A)
     dt = X.dtype
+    _check_index(yidx, batch, Y.shape[0], "yidx")
+    if X.shape[0] < batch:
+        raise ValueError(f"X needs {batch} rows, has {X.shape[0]}")
     mats = [[np.ascontiguousarray(A[b][k], dtype=dt) for k in range(d)] for b in range(batch)]
     Aptr = np.array([[_ptr(mats[b][k]) for k in range(d)] for b in range(batch)], dtype=np.uintp).reshape(batch, d)
     Xw = np.array(X, copy=True, order="C")

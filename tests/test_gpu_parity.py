@@ -300,6 +300,17 @@ def test_twopass_family_selected(km, d, n):
     assert km.query_kernel(d, n, 4)[0] == km.FAMILY_TWOPASS
 
 
+@pytest.mark.parametrize("d,n,batch", [(6, 7, 150), (6, 8, 70)])
+def test_twopass_large_batch(km, d, n, batch):
+    """Two-pass family over a batch whose intermediates (one n^d tensor per
+    entry in W, 140 MB at n = 7) far exceed the caches, outputs shared 2:1."""
+    A, X, Y0 = rng_inputs(900 + n, d, n, batch, batch // 2)
+    yidx = np.arange(batch) % (batch // 2)
+    want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
+    got = gpu_pbatched(A, X, Y0, yidx, with_w=True)
+    assert rel_err(got, want) <= RTOL_FP64
+
+
 def test_twopass_xbatched_chunked_by_subbatch(km):
     """d=6, n=7 through xbatched with subbatchCount=2: the two passes run in
     chunks of two entries, each using the workspace pair of its slot."""
