@@ -34,7 +34,7 @@ namespace km
 
 // fiber-loop unroll factors of the LDS stage passes (tuning knobs)
 #ifndef KM_LDS_UNROLL
-#define KM_LDS_UNROLL 2
+#define KM_LDS_UNROLL 4
 #endif
 #ifndef KM_SLAB_UNROLL
 #define KM_SLAB_UNROLL 1
