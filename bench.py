@@ -12,6 +12,11 @@ entry, test_kronmult6_batched.cpp:270), the HBM roofline of the kernel
 reference's CPU path timed on this machine's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode atomic|plan]
+                    [--workload c2a|c2b|d6n4|c4]
+
+--workload selects another uniform pbatched configuration for profiling
+(default c2a, the one above): c2b d=3 n=8 x262144 4:1, d6n4 d=6 n=4 x65536
+16:1, c4 d=6 n=8 x1024 2:1 (workspace kernels).
 """
 from __future__ import annotations
 
@@ -26,8 +31,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
-D, N, BATCH, OVERLAP = 6, 6, 4096, 4
 SEED = 20261203
+# name -> (d, n, batch, overlap): pY[k] = Y + (k/overlap)*n^d
+WORKLOADS = {
+    "c2a": (6, 6, 4096, 4),
+    "c2b": (3, 8, 262144, 4),
+    "d6n4": (6, 4, 65536, 16),
+    "c4": (6, 8, 1024, 2),
+}
 
 
 def flops_per_call(d, n, batch):
@@ -53,7 +64,7 @@ def parse_traffic(path):
     return None
 
 
-def cpu_baseline(A, X, yidx, n_y, reps=3):
+def cpu_baseline(A, X, yidx, n_y, overlap, reps=3):
     """The reference's own CPU/OpenMP path (oracle/_ref, default recursive
     build) when it was built, else the oracle restatement; timed on the host
     cores over the full workload (median of `reps`)."""
@@ -62,8 +73,7 @@ def cpu_baseline(A, X, yidx, n_y, reps=3):
     from oracle import oracle as O
     use_ref = "recursive" if O.ref_available("recursive") else None
     threads = O.num_threads()
-    d, n = D, N
-    batch = A.shape[0]
+    batch, d, n, _ = A.shape
     s = n ** d
     Y = np.zeros((n_y, s))
     times = []
@@ -80,8 +90,8 @@ def cpu_baseline(A, X, yidx, n_y, reps=3):
         "unit": "GFLOP/s",
         "cores": threads,
         "kind": "reference" if use_ref else "port",
-        "sample": f"full workload ({batch} entries, d={d}, n={n}, 4:1 overlap), median of {reps} "
-                  f"calls of kronmult6_pbatched (default recursive build), OpenMP threads={threads}",
+        "sample": f"full workload ({batch} entries, d={d}, n={n}, {overlap}:1 overlap), median of {reps} "
+                  f"calls of kronmult{d}_pbatched (default recursive build), OpenMP threads={threads}",
     }
 
 
@@ -96,6 +106,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2a",
+                    help="configuration (default c2a: d=6, n=6, 4096 entries, 4:1)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -110,18 +122,22 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    d, n, batch = D, N, BATCH
+    d, n, batch, overlap = WORKLOADS[args.workload]
     s = n ** d
-    n_y = batch // OVERLAP
+    n_y = batch // overlap
 
     g = torch.Generator(device=dev)
     g.manual_seed(SEED)
     A = torch.rand((batch, d, n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1
     X = torch.rand((batch, s), generator=g, device=dev, dtype=torch.float64) * 2 - 1
     Y = torch.zeros((n_y, s), device=dev, dtype=torch.float64)
-    yidx = np.arange(batch) // OVERLAP
+    yidx = np.arange(batch) // overlap
     pX = km.pointer_array(X, np.arange(batch))
     pY = km.pointer_array(Y, yidx)
+    fam, nt = km.query_kernel(d, n, 8, None)
+    # the workspace kernels (two-pass, generic) use one n^d slot per entry
+    W = torch.empty((batch, s), device=dev, dtype=torch.float64) if fam in (2, 3) else None
+    pW = km.pointer_array(W, np.arange(batch)) if W is not None else None
 
     # a plan is set-up work (once per output pointer array), outside the
     # timed region; every timed call does the full computation
@@ -129,14 +145,14 @@ def main():
     modes = {"atomic": None, "plan": km.options(plan=plan) if plan else None}
     todo = ["atomic", "plan"] if args.mode == "best" else [args.mode]
 
-    fam, nt = km.query_kernel(d, n, 8, None)
-    kernel_name = {0: "kron_lds", 1: "kron_slab", 2: "kron_global"}[fam]
+    kernel_name = {0: "kron_lds", 1: "kron_slab", 2: "kron_global", 3: "kron_sub+kron_pair",
+                   4: "kron_reg"}[fam]
 
     stream = torch.cuda.current_stream()
 
     def timed(opt):
         def step():
-            km.kronmult_pbatched(d, n, A, pX, pY, None, batch, stream=stream, opt=opt)
+            km.kronmult_pbatched(d, n, A, pX, pY, pW, batch, stream=stream, opt=opt)
 
         for _ in range(args.warmup):
             step()
@@ -156,7 +172,7 @@ def main():
     ms_per_step, wall = results[best]
 
     fl = flops_per_call(d, n, batch)
-    by = algorithmic_bytes(d, n, batch, OVERLAP)
+    by = algorithmic_bytes(d, n, batch, overlap)
     gflops = fl / (ms_per_step * 1e-3) / 1e9
     gbs = by / (ms_per_step * 1e-3) / 1e9
 
@@ -168,12 +184,12 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
 
-    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"{kernel_name}_d6n6_b4096.txt"))
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"{kernel_name}_d{d}n{n}_b{batch}.txt"))
 
     cpu = None
     if not args.no_cpu:
         # same seeded inputs, host copies
-        cpu = cpu_baseline(A.cpu().numpy(), X.cpu().numpy(), yidx, n_y)
+        cpu = cpu_baseline(A.cpu().numpy(), X.cpu().numpy(), yidx, n_y, overlap)
 
     out = {
         "metric": "d=6 batched kronmult GFLOP/s + % HBM roofline; mixed vs FP64 at rel err <=1e-10",
@@ -189,9 +205,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded U[-1,1) A and X on device)",
         "config": {
-            "workload": "kronmult6_pbatched FP64, d=6, n=6, batch=4096, 4:1 output overlap "
-                        "(pY[k] = Y + (k/4)*n^d)",
-            "d": d, "n": n, "batch": batch, "overlap": OVERLAP,
+            "workload": f"kronmult{d}_pbatched FP64, d={d}, n={n}, batch={batch}, {overlap}:1 output "
+                        f"overlap (pY[k] = Y + (k/{overlap})*n^d)",
+            "d": d, "n": n, "batch": batch, "overlap": overlap,
             "reduction": {"plan": "plan (deterministic output-sorted, no atomics)",
                           "atomic": "atomic adds (drop-in)"}[best],
             "ms_per_step_by_reduction": {m: round(v[0], 5) for m, v in results.items()},

@@ -241,7 +241,8 @@ km_status km_kronmult_grouped_f32(km_group_plan plan,
  * Introspection: which kernel family a square (d, n) call would use.
  *   *family: 0 = LDS-resident, 1 = slab-streamed, 2 = workspace-staged
  *            (generic), 3 = two-pass (sub-slabs in LDS, then the two
- *            leading modes in registers; uses W / pW).
+ *            leading modes in registers; uses W / pW), 4 = register-blocked
+ *            (mode groups in registers, LDS transposes between them).
  *   *threads: workgroup size.
  * ------------------------------------------------------------------------ */
 km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,

@@ -48,7 +48,7 @@ FLAG_PERSISTENT = 0x4
 FLAG_DETERMINISTIC = 0x8
 
 # kernel families reported by query_kernel
-FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL, FAMILY_TWOPASS = 0, 1, 2, 3
+FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL, FAMILY_TWOPASS, FAMILY_REG = 0, 1, 2, 3, 4
 
 # every symbol the C ABI exports (include/kronmult_amd.h)
 C_ABI_SYMBOLS = (

@@ -238,7 +238,8 @@ bool bin_kernel(int d, int n, bool exact, KernelInfo *k)
   Resolved r;
   r.exact = exact;
   r.fwd = false;
-  return pick_onchip<T>(d, n, r, k) && k->family == km::FAM_LDS;
+  return pick_onchip<T>(d, n, r, k) &&
+         (k->family == km::FAM_LDS || k->family == km::FAM_REG);
 }
 
 // Launch description common to all entry points.

@@ -15,7 +15,8 @@ enum Family
   FAM_SLAB = 1,
   FAM_GLOBAL = 2,
   FAM_TWOPASS = 3,
-  FAM_GROUPED = 4
+  FAM_REG = 4,
+  FAM_GROUPED = 5
 };
 
 struct KernelInfo

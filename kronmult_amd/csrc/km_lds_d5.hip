@@ -1,5 +1,6 @@
 // kron_lds instantiations for d = 5 (generated list: every n <= 8 whose
-// n^d tensor fits the 150 KiB LDS budget of kron_lds).
+// n^d tensor fits the 150 KiB LDS budget of kron_lds; float n = 8 fits but
+// would need 64 accumulator registers per thread, so it uses kron_slab).
 #include "km_internal.h"
 
 namespace km
@@ -33,7 +34,6 @@ bool lds_kernel_d<float, 5>(int n, bool exact, bool fwd, KernelInfo *out)
     KM_LDS_CASE(float, 5, 5)
     KM_LDS_CASE(float, 5, 6)
     KM_LDS_CASE(float, 5, 7)
-    KM_LDS_CASE(float, 5, 8)
   }
   return false;
 }

@@ -10,7 +10,7 @@ namespace km
     out->fn = exact ? (const void *)&kron_reg<T, D, N, RM, true>               \
                     : (const void *)&kron_reg<T, D, N, RM, false>;             \
     out->nt = cpow(N, D - RM);                                                 \
-    out->family = FAM_LDS;                                                     \
+    out->family = FAM_REG;                                                     \
     return true;                                                               \
   }
 

@@ -27,6 +27,7 @@ template<>
 bool slab_kernel<float>(int d, int n, bool exact, KernelInfo *out)
 {
   KM_SLAB_CASE(float, 6, 6, 512)
+  KM_SLAB_CASE(float, 5, 8, 512)
   return false;
 }
 

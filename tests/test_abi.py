@@ -94,8 +94,10 @@ def test_kron_flops_matches_reference_model():
 
 def test_query_kernel_families():
     import kronmult_amd as km
-    assert km.query_kernel(6, 4)[0] == km.FAMILY_LDS
-    assert km.query_kernel(3, 8)[0] == km.FAMILY_LDS
+    assert km.query_kernel(6, 4)[0] == km.FAMILY_REG
+    assert km.query_kernel(3, 8)[0] == km.FAMILY_REG
+    assert km.query_kernel(6, 5)[0] == km.FAMILY_LDS
+    assert km.query_kernel(6, 4, opt=km.options(exact=True))[0] == km.FAMILY_LDS   # backward order
     assert km.query_kernel(6, 6)[0] == km.FAMILY_SLAB
     assert km.query_kernel(5, 8)[0] == km.FAMILY_SLAB
     assert km.query_kernel(6, 8)[0] == km.FAMILY_TWOPASS
