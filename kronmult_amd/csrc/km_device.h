@@ -707,7 +707,10 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
       target = yb;
     }
     T pf = 0;
-    if (p + 1 < p1)
+    // the next entry's A into L2: pays for 8x8 factors (d3n8 0.53 -> 0.49 ms),
+    // costs for 4x4 ones, whose few lines are cheap to load on demand
+    // (d6n4 0.57 -> 0.64 ms with it)
+    if (N >= 8 && p + 1 < p1)
       pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
 #pragma unroll
     for (int g = 0; g < NG; g++)
