@@ -351,6 +351,14 @@ __device__ __forceinline__ T prefetch_a(const EntryArgs<T> &a, int b, int tid)
   return v;
 }
 
+// Whether kron_lds / kron_reg pull the next entry's A into L2 during the
+// current entry.  Measured: it pays for 7x7 and 8x8 factors (d2n8 1.43 ->
+// 1.10 ms, d3n8 0.53 -> 0.49 ms, d5n7 0.223 -> 0.207 ms) and costs for n <= 6,
+// whose few lines are cheaper to load on demand (d6n4 0.64 -> 0.57 ms,
+// d3n4 0.83 -> 0.73 ms, d6n3 0.35 -> 0.31 ms without it).
+template<int N>
+constexpr bool kPrefetchA = N >= 7;
+
 template<typename T>
 __device__ __forceinline__ void keep_alive(T v)
 {
@@ -479,7 +487,7 @@ __device__ __forceinline__ void lds_body(const EntryArgs<T> &a, T *buf, int p0,
       }
     }
     T pf = 0;
-    if (p + 1 < p1)
+    if (kPrefetchA<N> && p + 1 < p1)
       pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
     if (yb != target)
     {
@@ -707,10 +715,7 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
       target = yb;
     }
     T pf = 0;
-    // the next entry's A into L2: pays for 8x8 factors (d3n8 0.53 -> 0.49 ms),
-    // costs for 4x4 ones, whose few lines are cheap to load on demand
-    // (d6n4 0.57 -> 0.64 ms with it)
-    if (N >= 8 && p + 1 < p1)
+    if (kPrefetchA<N> && p + 1 < p1)
       pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
 #pragma unroll
     for (int g = 0; g < NG; g++)
