@@ -12,11 +12,11 @@ entry, test_kronmult6_batched.cpp:270), the HBM roofline of the kernel
 reference's CPU path timed on this machine's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode atomic|plan]
-                    [--workload c2a|c2b|d6n4|c4]
+                    [--workload d6n6|d3n8|d6n4|d6n8]
 
 --workload selects another uniform pbatched configuration for profiling
-(default c2a, the one above): c2b d=3 n=8 x262144 4:1, d6n4 d=6 n=4 x65536
-16:1, c4 d=6 n=8 x1024 2:1 (workspace kernels).
+(default d6n6, the one above): d3n8 = d=3 n=8 x262144 4:1, d6n4 = d=6 n=4
+x65536 16:1, d6n8 = d=6 n=8 x1024 2:1 (workspace kernels).
 """
 from __future__ import annotations
 
@@ -34,10 +34,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 SEED = 20261203
 # name -> (d, n, batch, overlap): pY[k] = Y + (k/overlap)*n^d
 WORKLOADS = {
-    "c2a": (6, 6, 4096, 4),
-    "c2b": (3, 8, 262144, 4),
+    "d6n6": (6, 6, 4096, 4),
+    "d3n8": (3, 8, 262144, 4),
     "d6n4": (6, 4, 65536, 16),
-    "c4": (6, 8, 1024, 2),
+    "d6n8": (6, 8, 1024, 2),
 }
 
 
@@ -106,8 +106,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2a",
-                    help="configuration (default c2a: d=6, n=6, 4096 entries, 4:1)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="d6n6",
+                    help="configuration (default d6n6: d=6, n=6, 4096 entries, 4:1)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -196,7 +196,7 @@ def test_plan_deterministic_and_correct(km, d, n, pattern):
     nY = 8 if pattern != "random" else 5
     A, X, Y0 = rng_inputs(700 + d + n, d, n, batch, nY)
     if pattern == "mod":
-        yidx = np.arange(batch) % nY           # C1-style: k mod nY
+        yidx = np.arange(batch) % nY           # interleaved outputs: k mod nY
     elif pattern == "block":
         yidx = np.arange(batch) * nY // batch  # C2-style: consecutive blocks
     else:
@@ -230,7 +230,7 @@ def test_plan_mismatch_rejected(km):
     plan.close()
 
 
-def test_identity_full_size_c2a(km):
+def test_identity_full_size_d6n6_b4096(km):
     """A = I at the full d=6, n=6, 4096-entry shape with 4:1 output overlap:
     Y must equal Y0 + sum of the X of its four entries exactly (each Y
     element receives four additions of exactly representable values)."""
@@ -254,7 +254,7 @@ def test_identity_full_size_c2a(km):
     assert torch.equal(Y, want)
 
 
-def test_linearity_full_size_c2b(km):
+def test_linearity_full_size_d3n8_b262144(km):
     """d=3, n=8, 262144 entries, 4:1 overlap: kron(A)(X1 + X2) equals
     kron(A)X1 + kron(A)X2 to rounding (size-independent property)."""
     import torch

@@ -1,4 +1,4 @@
-"""Precision study at the C4 shape (d = 6, n = 8, outputs shared 2:1; A with
+"""Precision study at d = 6, n = 8 (outputs shared 2:1; A with
 entries N(0,1)/sqrt(8) and one row per matrix scaled by 1e3, X ~ U[-1,1]),
 on a batch the CPU oracle finishes in seconds.
 
@@ -26,7 +26,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def c4_inputs(batch: int, seed: int = 4):
+def scaled_row_inputs(batch: int, seed: int = 4):
     d, n = 6, 8
     rng = np.random.default_rng(seed)
     A = rng.standard_normal((batch, d, n, n)) / np.sqrt(8.0)
@@ -55,9 +55,9 @@ def km():
     return kronmult_amd
 
 
-def test_c4_precision_study(km):
+def test_precision_study_d6n8_scaled_rows(km):
     batch = 16
-    A, X, Y0, yidx = c4_inputs(batch)
+    A, X, Y0, yidx = scaled_row_inputs(batch)
     want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
     out = {}
     fma = gpu_pbatched(A, X, Y0, yidx, with_w=True)
@@ -70,10 +70,10 @@ def test_c4_precision_study(km):
     out["float_inputs"] = _stats(f32.astype(np.float64), want)
     assert out["float_inputs"]["normwise"] <= RTOL_FP32
     # single-entry outputs: EXACT reproduces the oracle bitwise
-    A1, X1, Y1, _ = c4_inputs(4, seed=5)
+    A1, X1, Y1, _ = scaled_row_inputs(4, seed=5)
     y1 = np.arange(4)
     Y1 = np.zeros((4, X1.shape[1]))
     w1 = O.pbatched(A1, X1, Y1, y1, variant=O.BACKWARD)
     g1 = gpu_pbatched(A1, X1, Y1, y1, with_w=True, opt=km.options(exact=True))
     assert np.array_equal(g1, w1)
-    print("C4 precision study (16 entries, 2:1):", json.dumps(out))
+    print("precision study d=6 n=8 (16 entries, 2:1):", json.dumps(out))
