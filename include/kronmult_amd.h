@@ -213,7 +213,9 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
  * resident workgroups, outputs shared between workgroups accumulated with
  * atomics.  KM_FLAG_DETERMINISTIC splits each bin at output boundaries
  * instead, so every output is owned by one workgroup (no atomics, bitwise
- * reproducible); KM_FLAG_PERSISTENT runs all bins in a single persistent
+ * reproducible); its bins run concurrently on up to four streams the plan
+ * owns, forked from `stream` and joined back into it with events (also
+ * capturable), so a plan serves one call at a time; KM_FLAG_PERSISTENT runs all bins in a single persistent
  * launch with owned outputs.  Creating the plan reads the host array n_host
  * and the device pointer array pY once; calls must pass the same pY
  * contents.

@@ -56,6 +56,11 @@ struct km_group_plan_s
   // per-size-bin launch: bin b uses binranges[first .. first+count]
   std::vector<km_group_bin> bins;
   int *binranges = nullptr; // device
+  // bins run concurrently on these streams (default priority), forked from
+  // and joined back into the caller's stream with events
+  std::vector<hipStream_t> streams;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
 };
 
 namespace
@@ -614,11 +619,28 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     a.pX = pX;
     a.pY = pY;
     const bool owned = (o.flags & KM_FLAG_DETERMINISTIC) != 0;
-    for (const km_group_bin &bn : plan->bins)
-    {
-      KernelInfo kb;
-      if (!bin_kernel<T>(plan->d, bn.n, exact, &kb))
+    // resolve every bin's kernel before the fork, so an unsupported size
+    // returns with nothing launched and no stream left forked
+    std::vector<KernelInfo> kbs(plan->bins.size());
+    for (size_t i = 0; i < plan->bins.size(); i++)
+      if (!bin_kernel<T>(plan->d, plan->bins[i].n, exact, &kbs[i]))
         return KM_ERROR_NOT_SUPPORTED;
+    // owned outputs: a bin's grid is its number of output runs, often far
+    // below a full chip, so the bins run concurrently on the plan's streams.
+    // The even split of the atomic path already sizes each bin's grid to
+    // fill the chip, and there the fork and join only add latency.
+    const int ns = owned ? (int)plan->streams.size() : 0;
+    if (ns > 1)
+    {
+      KM_HIP(hipEventRecord(plan->fork, (hipStream_t)stream));
+      for (hipStream_t st : plan->streams)
+        KM_HIP(hipStreamWaitEvent(st, plan->fork, 0));
+    }
+    for (size_t bi = 0; bi < plan->bins.size(); bi++)
+    {
+      const km_group_bin &bn = plan->bins[bi];
+      const KernelInfo &kb = kbs[bi];
+      hipStream_t bs = ns > 1 ? plan->streams[bi % ns] : (hipStream_t)stream;
       int grid;
       if (owned)
       {
@@ -642,8 +664,15 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
         grid = (bn.cnt + bn.chunk - 1) / bn.chunk;
       }
       void *args[] = {&a};
-      KM_HIP(hipLaunchKernel(kb.fn, dim3(grid), dim3(kb.nt), args, 0,
-                             (hipStream_t)stream));
+      KM_HIP(hipLaunchKernel(kb.fn, dim3(grid), dim3(kb.nt), args, 0, bs));
+    }
+    if (ns > 1)
+    {
+      for (int i = 0; i < ns; i++)
+      {
+        KM_HIP(hipEventRecord(plan->joins[i], plan->streams[i]));
+        KM_HIP(hipStreamWaitEvent((hipStream_t)stream, plan->joins[i], 0));
+      }
     }
     return KM_SUCCESS;
   }
@@ -1194,6 +1223,23 @@ extern "C"
                        hipMemcpyHostToDevice));
       KM_HIP(hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
                        hipMemcpyHostToDevice));
+      // different sizes never share an output, so the size bins may run
+      // concurrently: up to four streams, bins dealt out largest first
+      // (used by the owned-output path)
+      const int ns = std::min<int>(4, (int)p->bins.size());
+      if (ns > 1)
+      {
+        KM_HIP(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+        for (int i = 0; i < ns; i++)
+        {
+          hipStream_t st = nullptr;
+          KM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+          p->streams.push_back(st);
+          hipEvent_t ev = nullptr;
+          KM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+          p->joins.push_back(ev);
+        }
+      }
       return KM_SUCCESS;
     }();
     if (st2 != KM_SUCCESS)
@@ -1217,6 +1263,12 @@ extern "C"
       (void)hipFree(plan->ranges);
     if (plan->binranges)
       (void)hipFree(plan->binranges);
+    for (hipStream_t st : plan->streams)
+      (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : plan->joins)
+      (void)hipEventDestroy(ev);
+    if (plan->fork)
+      (void)hipEventDestroy(plan->fork);
     delete plan;
     return KM_SUCCESS;
   }

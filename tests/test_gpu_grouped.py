@@ -122,3 +122,47 @@ def test_grouped_rejects_oversize(km):
     with pytest.raises(km.KronmultError) as ei:
         km.GroupPlan(6, [4, 5], pY)
     assert ei.value.status == km.ERROR_NOT_SUPPORTED
+
+
+@pytest.mark.parametrize("mode", ["atomic", "deterministic"])
+def test_grouped_graph_capture_replays(km, mode):
+    """The per-size launches fork onto the plan's streams and join back into
+    the caller's stream; captured in a HIP graph (torch.cuda.graph) and
+    replayed with eager work in between, every replay matches the eager call
+    (bitwise with owned outputs) and the oracle."""
+    import torch
+    dev = torch.device("cuda:0")
+    d, sizes = 3, [2, 4, 6, 8, 5]
+    ns, yg, mats, X, Y0 = _make(31, d, sizes, 400, 6)
+    want = _oracle(d, ns, yg, mats, X, Y0)
+    tY0 = {n: torch.tensor(v, device=dev) for n, v in Y0.items()}
+    tY = {n: t.clone() for n, t in tY0.items()}
+    tX = [torch.tensor(x, device=dev) for x in X]
+    tA = [torch.tensor(np.ascontiguousarray(mats[b][k].T), device=dev) for b in range(len(ns)) for k in range(d)]
+    table = torch.tensor([t.data_ptr() for t in tA], dtype=torch.int64, device=dev)
+    pX = torch.tensor([t.data_ptr() for t in tX], dtype=torch.int64, device=dev)
+    pY = torch.tensor([tY[n][yg[b]].data_ptr() for b, n in enumerate(ns)], dtype=torch.int64, device=dev)
+    plan = km.GroupPlan(d, list(ns), pY)
+    opt = km.options(**MODES[mode])
+    km.kronmult_grouped(plan, table, pX, pY, opt=opt)  # eager
+    torch.cuda.synchronize()
+    eager = {n: (tY[n] - tY0[n]).clone() for n in tY}
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        km.kronmult_grouped(plan, table, pX, pY, opt=opt)
+    for rep in range(4):
+        for n in tY:
+            tY[n].copy_(tY0[n])
+        junk = torch.ones(1 << 20, device=dev, dtype=torch.float64).sum()  # eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        assert junk.item() == float(1 << 20)
+        for n in tY:
+            got = tY[n] - tY0[n]
+            if mode == "deterministic":
+                assert torch.equal(got, eager[n]), (rep, n)
+            else:
+                assert torch.allclose(got, eager[n], rtol=0, atol=1e-12 * float(eager[n].abs().max())), (rep, n)
+            assert rel_err(tY[n].cpu().numpy(), want[n]) <= RTOL_FP64, (rep, n)
+    del g
+    plan.close()
