@@ -247,6 +247,16 @@ bool bin_kernel(int d, int n, bool exact, KernelInfo *k)
          (k->family == km::FAM_LDS || k->family == km::FAM_REG);
 }
 
+// Whether a call's X vectors, each read once, are loaded with the
+// non-temporal hint (EntryArgs::x_stream).  A call whose X fits well inside
+// the 256 MB memory-side cache keeps the default policy: a repeated call
+// finds X there.
+constexpr long long kStreamBytes = 64ll << 20;
+inline int x_streams(long long entries, long long xsize, int elem_bytes)
+{
+  return entries * xsize * elem_bytes >= kStreamBytes ? 1 : 0;
+}
+
 // Launch description common to all entry points.
 template<typename T>
 struct Call
@@ -459,6 +469,7 @@ km_status batched(int d, int n, const T *A, const T *X, T *Y, T *W, int batch,
   c.a.w_half = 0;
   c.a.ws_mul = 1;
   c.a.slot_by_entry = 1;
+  c.a.x_stream = x_streams(batch, xs, sizeof(T));
   c.a.batch = batch;
   c.slots = W ? batch : 0;
   return launch(c, r, (hipStream_t)stream);
@@ -489,6 +500,7 @@ km_status pbatched(int d, int n, const T *A, T *const *pX, T *const *pY,
   c.a.pW = pW;
   c.a.ws_mul = 1;
   c.a.slot_by_entry = 1;
+  c.a.x_stream = x_streams(batch, c.xsize, sizeof(T));
   c.a.batch = batch;
   c.slots = pW ? batch : 0;
   return launch(c, r, (hipStream_t)stream);
@@ -520,6 +532,7 @@ km_status xbatched(int d, int n, const T *const *Aarray, int lda, T *const *pX,
   c.a.pY = pY;
   c.a.pW = pW;
   c.a.ws_mul = 2; // slot s uses the pair pW[2s], pW[2s+1]
+  c.a.x_stream = 0; // pooled inputs: entries share X vectors
   c.a.batch = batch;
   c.slots = pW ? sub : 0;
   return launch(c, r, (hipStream_t)stream);
@@ -582,6 +595,7 @@ km_status vbatched(int d, const int *m, const int *n, const T *const *Aarray,
   c.a.W = W;
   c.a.w_stride = 2 * sxw;
   c.a.w_half = sxw;
+  c.a.x_stream = x_streams(batch, sx, sizeof(T));
   c.a.batch = batch;
   c.slots = W ? (long long)(Wcap / (2 * sxw * sizeof(T))) : 0;
   return launch(c, r, (hipStream_t)stream);
@@ -618,6 +632,10 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     a.lda = 0;
     a.pX = pX;
     a.pY = pY;
+    long long xtotal = 0;
+    for (const km_group_bin &bn : plan->bins)
+      xtotal += (long long)bn.cnt * ipow(bn.n, plan->d);
+    a.x_stream = x_streams(1, xtotal, sizeof(T));
     const bool owned = (o.flags & KM_FLAG_DETERMINISTIC) != 0;
     // resolve every bin's kernel before the fork, so an unsupported size
     // returns with nothing launched and no stream left forked

@@ -74,6 +74,7 @@ struct EntryArgs
   const int *ranges;  // workgroup w walks order positions [ranges[w], ranges[w+1])
   const int *order;   // order position -> entry
   int owned;          // 1: workgroup owns its outputs (no atomics)
+  int x_stream;       // 1: every X is read once per call (non-temporal loads)
   int slot_by_entry;  // two-pass family: workspace slot = entry index (1) or
                       // position inside the launched chunk (0)
 };
@@ -143,6 +144,21 @@ template<typename T>
 __device__ __forceinline__ int entry_at(const EntryArgs<T> &a, int p)
 {
   return a.order ? cld(a.order + p) : p;
+}
+
+// X element load.  When every X vector is read once per call (batched,
+// pbatched, vbatched, grouped: EntryArgs::x_stream) the load carries the
+// non-temporal hint, so the streamed vectors do not push reused lines (A, the
+// next entries' prefetched A, Y) out of L2; pooled X (xbatched) keeps the
+// default policy, since entries share vectors there.  A compile-time choice:
+// the compiler merges a runtime select of the two loads into one plain load.
+template<bool XNT, typename T>
+__device__ __forceinline__ T gld_x(const T *p)
+{
+  if constexpr (XNT)
+    return __builtin_nontemporal_load((gptr<const T>)(p));
+  else
+    return *(gptr<const T>)(p);
 }
 
 template<typename T>
@@ -439,7 +455,8 @@ __device__ __forceinline__ void lds_stage(T *buf, const EntryArgs<T> &a, int b,
     lds_stage<T, D, N, NT, EXACT, FWD, K + 1, WAVE>(buf, a, b, tid);
 }
 
-template<typename T, int D, int N, int NT, bool EXACT, bool FWD, bool PRELOAD>
+template<typename T, int D, int N, int NT, bool EXACT, bool FWD, bool PRELOAD,
+         bool XNT>
 __device__ __forceinline__ void lds_body(const EntryArgs<T> &a, T *buf, int p0,
                                          int p1)
 {
@@ -459,7 +476,7 @@ __device__ __forceinline__ void lds_body(const EntryArgs<T> &a, T *buf, int p0,
     {
       int q = tid + NT * r;
       if (S % NT == 0 || q < S)
-        xr[r] = gld(xp + q);
+        xr[r] = gld_x<XNT>(xp + q);
     }
   }
 
@@ -483,7 +500,7 @@ __device__ __forceinline__ void lds_body(const EntryArgs<T> &a, T *buf, int p0,
       {
         int q = tid + NT * r;
         if (S % NT == 0 || q < S)
-          xr[r] = gld(xp + q);
+          xr[r] = gld_x<XNT>(xp + q);
       }
     }
     T pf = 0;
@@ -518,12 +535,22 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
   wg_range(a, p0, p1);
   if (p0 >= p1)
     return;
-  // owned runs with FMAs start from Y (see run_init); one specialised body
-  // per case keeps each loop's code as tight as a kernel of its own
-  if (!EXACT && a.owned)
-    lds_body<T, D, N, NT, EXACT, FWD, !EXACT>(a, buf, p0, p1);
+  // owned runs with FMAs start from Y (see run_init), streamed X is loaded
+  // non-temporally (gld_x); one specialised body per case keeps each loop's
+  // code as tight as a kernel of its own.  EXACT keeps one plain body.
+  if constexpr (EXACT)
+    lds_body<T, D, N, NT, EXACT, FWD, false, false>(a, buf, p0, p1);
+  else if (a.owned)
+  {
+    if (a.x_stream)
+      lds_body<T, D, N, NT, EXACT, FWD, true, true>(a, buf, p0, p1);
+    else
+      lds_body<T, D, N, NT, EXACT, FWD, true, false>(a, buf, p0, p1);
+  }
+  else if (a.x_stream)
+    lds_body<T, D, N, NT, EXACT, FWD, false, true>(a, buf, p0, p1);
   else
-    lds_body<T, D, N, NT, EXACT, FWD, false>(a, buf, p0, p1);
+    lds_body<T, D, N, NT, EXACT, FWD, false, false>(a, buf, p0, p1);
 }
 
 // ---------------------------------------------------------------------------
@@ -653,7 +680,7 @@ __device__ __forceinline__ int reg_pad(int idx)
   return idx + idx / RR;
 }
 
-template<typename T, int D, int N, int RM, bool EXACT, bool PRELOAD>
+template<typename T, int D, int N, int RM, bool EXACT, bool PRELOAD, bool XNT>
 __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
                                          int p1)
 {
@@ -703,7 +730,7 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
     const T *xp = x_of(a, b);
 #pragma unroll
     for (int r = 0; r < RR; r++)
-      v[r] = static_cast<C>(gld(xp + r * NTH + tid));
+      v[r] = static_cast<C>(gld_x<XNT>(xp + r * NTH + tid));
     T *yb = y_of(a, b);
     if (yb != target)
     {
@@ -749,10 +776,19 @@ __global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
   wg_range(a, p0, p1);
   if (p0 >= p1)
     return;
-  if (!EXACT && a.owned)
-    reg_body<T, D, N, RM, EXACT, !EXACT>(a, buf, p0, p1);
+  if constexpr (EXACT)
+    reg_body<T, D, N, RM, EXACT, false, false>(a, buf, p0, p1);
+  else if (a.owned)
+  {
+    if (a.x_stream)
+      reg_body<T, D, N, RM, EXACT, true, true>(a, buf, p0, p1);
+    else
+      reg_body<T, D, N, RM, EXACT, true, false>(a, buf, p0, p1);
+  }
+  else if (a.x_stream)
+    reg_body<T, D, N, RM, EXACT, false, true>(a, buf, p0, p1);
   else
-    reg_body<T, D, N, RM, EXACT, false>(a, buf, p0, p1);
+    reg_body<T, D, N, RM, EXACT, false, false>(a, buf, p0, p1);
 }
 
 // ---------------------------------------------------------------------------
@@ -768,7 +804,7 @@ typedef __attribute__((address_space(1))) void *gbl_vptr;
 // Copy SL elements (SL*sizeof(T) a multiple of 16, src 16-byte aligned) from
 // global memory straight into LDS, 16 bytes per lane.  Each wave writes one
 // contiguous 1 KiB block per instruction.
-template<typename T, int SL, int NT>
+template<typename T, int SL, int NT, bool XNT>
 __device__ __forceinline__ void slab_async_load(T *dst, const T *src, int tid)
 {
   if (reinterpret_cast<uintptr_t>(src) & 15)
@@ -793,7 +829,9 @@ __device__ __forceinline__ void slab_async_load(T *dst, const T *src, int tid)
       {
         const char *g = reinterpret_cast<const char *>(src) + (size_t)c * 16;
         char *l = reinterpret_cast<char *>(dst) + (size_t)c0 * 16;
-        __builtin_amdgcn_global_load_lds((gbl_vptr)(g), (lds_vptr)(l), 16, 0, 0);
+        // cache-policy immediate: 2 = non-temporal
+        __builtin_amdgcn_global_load_lds((gbl_vptr)(g), (lds_vptr)(l), 16, 0,
+                                         XNT ? 2 : 0);
       }
     }
   }
@@ -823,7 +861,7 @@ __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
     slab_stage<T, D, N, NT, EXACT, K + 1>(buf, a, b, tid);
 }
 
-template<typename T, int D, int N, int NT, bool EXACT, bool PRELOAD>
+template<typename T, int D, int N, int NT, bool EXACT, bool PRELOAD, bool XNT>
 __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
                                           int p0, int p1)
 {
@@ -838,14 +876,32 @@ __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
 
   // slab sequence: (entry p, j1) flattened as t = (p - p0)*N + j1
   const int nslab = (p1 - p0) * N;
+  // Workgroups run in near lockstep, so with equal output runs they would
+  // all flush at the same moments, and HBM would see bursts of atomic Y
+  // traffic between quiet stretches.  With shared (atomic) outputs each
+  // workgroup therefore starts its range at position blockIdx % count and
+  // wraps around: the run it starts inside is flushed twice, and the flushes
+  // of neighbouring workgroups fall at different times.  Owned runs keep the
+  // position order (a second flush of an owned run costs a Y read-modify-
+  // write and measured slower).
+  const int cnt = p1 - p0;
+  const int rot = (PRELOAD || a.owned) ? 0 : (int)(blockIdx.x % (unsigned)cnt);
+  auto pos = [&](int q) {
+    if constexpr (!PRELOAD)
+    {
+      q += rot;
+      q = q >= cnt ? q - cnt : q;
+    }
+    return p0 + q;
+  };
   {
-    const T *xp = x_of(a, entry_at(a, p0));
-    slab_async_load<T, SL, NT>(sb0, xp, tid);
+    const T *xp = x_of(a, entry_at(a, pos(0)));
+    slab_async_load<T, SL, NT, XNT>(sb0, xp, tid);
   }
 
   for (int t = 0; t < nslab; ++t)
   {
-    const int p = p0 + t / N;
+    const int p = pos(t / N);
     const int j1 = t - (t / N) * N;
     const int b = entry_at(a, p);
     T *cur = (t & 1) ? sb1 : sb0;
@@ -875,10 +931,10 @@ __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
     }
     if (t + 1 < nslab)
     {
-      const int pn = p0 + (t + 1) / N;
+      const int pn = pos((t + 1) / N);
       const int jn = (t + 1) - ((t + 1) / N) * N;
       const T *xp = x_of(a, entry_at(a, pn));
-      slab_async_load<T, SL, NT>(nxt, xp + (size_t)jn * SL, tid);
+      slab_async_load<T, SL, NT, XNT>(nxt, xp + (size_t)jn * SL, tid);
     }
 
     slab_stage<T, D, N, NT, EXACT, 0>(cur, a, b, tid);
@@ -917,10 +973,16 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
   wg_range(a, p0, p1);
   if (p0 >= p1)
     return;
-  if (!EXACT && a.owned)
-    slab_body<T, D, N, NT, EXACT, !EXACT>(a, sb0, sb1, p0, p1);
+  // owned FMA runs start from Y; shared outputs of streamed X use
+  // non-temporal slab loads (with owned runs those measured slower)
+  if constexpr (EXACT)
+    slab_body<T, D, N, NT, EXACT, false, false>(a, sb0, sb1, p0, p1);
+  else if (a.owned)
+    slab_body<T, D, N, NT, EXACT, true, false>(a, sb0, sb1, p0, p1);
+  else if (a.x_stream)
+    slab_body<T, D, N, NT, EXACT, false, true>(a, sb0, sb1, p0, p1);
   else
-    slab_body<T, D, N, NT, EXACT, false>(a, sb0, sb1, p0, p1);
+    slab_body<T, D, N, NT, EXACT, false, false>(a, sb0, sb1, p0, p1);
 }
 
 // ---------------------------------------------------------------------------

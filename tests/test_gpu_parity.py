@@ -188,6 +188,40 @@ def test_float_entry_points(km, d, n):
     assert rel_err(got, want64) <= RTOL_FP32
 
 
+@pytest.mark.parametrize("d,n", [(4, 5), (6, 6), (5, 8), (6, 4)])
+def test_float_subnormal_and_scaled_factors(km, d, n):
+    """Float factors are widened to double on their bit patterns.  Of the
+    three last modes, the middle one is scaled by 2^-140 (every entry a float
+    subnormal) and its neighbours by 2^70, so the intermediates stay normal
+    floats in either mode order and the net scale is 1: a mis-widened
+    subnormal or exponent shows as an O(1) error against the oracle, which
+    widens the same floats."""
+    batch = 4
+    A, X, Y0 = rng_inputs(810 + d * n, d, n, batch, 2, dtype=np.float32)
+    A = A.astype(np.float64)
+    A[:, d - 1] *= 2.0 ** 70
+    A[:, d - 2] *= 2.0 ** -140
+    A[:, d - 3] *= 2.0 ** 70
+    A = A.astype(np.float32)
+    assert np.all(np.abs(A[:, d - 2]) < np.finfo(np.float32).tiny)
+    yidx = np.arange(batch) % 2
+    want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
+    got = gpu_pbatched(A, X, Y0, yidx, with_w=True)
+    assert rel_err(got, want) <= RTOL_FP32
+
+
+def test_float_zero_and_infinite_factors(km):
+    """+-0 and +-inf float factors (d = 1, n = 3): infinities and NaNs land
+    where the oracle puts them, finite outputs match."""
+    A = np.array([[[[np.inf, 0.0, 1.0], [-0.0, 1e-40, 2.0], [1.0, -np.inf, 3.0]]]], dtype=np.float32)
+    X = np.array([[1.0, 2.0, 0.5]], dtype=np.float32)
+    Y0 = np.zeros((1, 3), dtype=np.float32)
+    want = O.pbatched(A, X, Y0, np.zeros(1, dtype=np.int64), variant=O.BACKWARD)
+    got = gpu_pbatched(A, X, Y0, np.zeros(1, dtype=np.int64))
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    np.testing.assert_allclose(got[~np.isnan(got)], want[~np.isnan(want)], rtol=1e-6)
+
+
 @pytest.mark.parametrize("d,n,pattern", [(6, 4, "mod"), (6, 4, "block"), (3, 8, "block"),
                                          (6, 6, "block"), (6, 6, "mod"), (4, 5, "random")])
 def test_plan_deterministic_and_correct(km, d, n, pattern):
@@ -230,10 +264,36 @@ def test_plan_mismatch_rejected(km):
     plan.close()
 
 
-def test_identity_full_size_d6n6_b4096(km):
+@pytest.mark.parametrize("d,n,batch", [(6, 4, 2048), (5, 8, 256), (6, 5, 540), (3, 8, 16384)])
+@pytest.mark.parametrize("mode", ["atomic", "plan"])
+def test_streamed_inputs_vs_oracle(km, d, n, batch, mode):
+    """Calls whose X totals at least 64 MiB load X with the non-temporal
+    hint (kron_reg, kron_lds; kron_slab with shared outputs), in separately
+    compiled bodies: each against the oracle, with 4:1 output overlap."""
+    import torch
+    A, X, Y0 = rng_inputs(950 + d * n, d, n, batch, batch // 4)
+    assert X.nbytes >= 64 << 20
+    yidx = np.arange(batch) // 4
+    opt = None
+    plan = None
+    if mode == "plan":
+        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
+        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
+        opt = km.options(plan=plan)
+    got = gpu_pbatched(A, X, Y0, yidx, opt=opt)
+    want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
+    assert rel_err(got, want) <= RTOL_FP64
+    if plan is not None:
+        plan.close()
+
+
+@pytest.mark.parametrize("mode", ["atomic", "plan"])
+def test_identity_full_size_d6n6_b4096(km, mode):
     """A = I at the full d=6, n=6, 4096-entry shape with 4:1 output overlap:
     Y must equal Y0 + sum of the X of its four entries exactly (each Y
-    element receives four additions of exactly representable values)."""
+    element receives four additions of exactly representable values).  With
+    atomic outputs each workgroup starts its range part-way into an output
+    run, so that run is flushed in two pieces."""
     import torch
     d, n, batch = 6, 6, 4096
     dev = torch.device("cuda:0")
@@ -248,10 +308,13 @@ def test_identity_full_size_d6n6_b4096(km):
     Y = torch.zeros((batch // 4, s), dtype=torch.float64, device=dev)
     pX = km.pointer_array(X, np.arange(batch))
     pY = km.pointer_array(Y, np.arange(batch) // 4)
-    km.kronmult_pbatched(d, n, A, pX, pY, None, batch)
+    plan = km.Plan(d, n, 8, pY, batch) if mode == "plan" else None
+    km.kronmult_pbatched(d, n, A, pX, pY, None, batch, opt=km.options(plan=plan))
     torch.cuda.synchronize()
     want = X.view(batch // 4, 4, s).sum(dim=1)
     assert torch.equal(Y, want)
+    if plan is not None:
+        plan.close()
 
 
 def test_linearity_full_size_d3n8_b262144(km):
