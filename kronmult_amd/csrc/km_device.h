@@ -212,17 +212,40 @@ struct Acc<float>
   using type = double;
 };
 
+// A wave-uniform double held in VGPRs (a float widened by the vector unit)
+// moved to an SGPR pair.  The compiler folds its own readfirstlane of a value
+// it knows to be uniform and then keeps the double in two VGPRs per lane.
+__device__ __forceinline__ double to_sgpr(double v)
+{
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  unsigned lo, hi;
+  asm volatile("v_readfirstlane_b32 %0, %1" : "=s"(lo) : "v"((unsigned)u));
+  asm volatile("v_readfirstlane_b32 %0, %1" : "=s"(hi) : "v"((unsigned)(u >> 32)));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Load A_k (n x n, leading dimension lda) into registers.  The pointer is
 // wave-uniform, so these become scalar loads and the values live in SGPRs.
-template<int N, typename T>
+// Float factors are widened by the vector unit and stay in VGPRs unless
+// SGPR_WIDE moves them back to SGPRs: kron_slab at n <= 6 needs that (its
+// double accumulator leaves no room for 2*n*n more VGPRs; float d6n6 1.12 ->
+// 0.86 ms); the other kernels, and n = 8, measured slower with it.
+template<int N, bool SGPR_WIDE = false, typename T>
 __device__ __forceinline__ void load_a(typename Acc<T>::type (&am)[N][N],
                                        const T *Ak, int lda)
 {
+  using C = typename Acc<T>::type;
 #pragma unroll
   for (int j = 0; j < N; j++)
 #pragma unroll
     for (int i = 0; i < N; i++)
-      am[i][j] = static_cast<typename Acc<T>::type>(cld(Ak + (i + j * lda)));
+    {
+      const C v = static_cast<C>(cld(Ak + (i + j * lda)));
+      if constexpr (SGPR_WIDE && sizeof(C) != sizeof(T))
+        am[i][j] = to_sgpr(v);
+      else
+        am[i][j] = v;
+    }
 }
 
 // out_i = sum_j am[i][j] * x_j over j ascending (started from zero as in the
@@ -582,7 +605,7 @@ __device__ __forceinline__ void reg_group(typename Acc<T>::type (&v)[cpow(N, RM)
   {
     // all of A_k fits in SGPRs: contract each fiber in place
     C am[N][N];
-    load_a<N, T>(am, Ak, lda);
+    load_a<N, false, T>(am, Ak, lda);
 #pragma unroll
     for (int base = 0; base < RR; base++)
     {
@@ -848,7 +871,8 @@ __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
   constexpr int MODE = D - 1 - K;
   constexpr int STRIDE = cpow(N, D - 1 - MODE);
   C am[N][N];
-  load_a<N>(am, a_of(a, b, MODE, D), a.lda);
+  // float: widened A back in SGPRs where n*n doubles fit there (n <= 6)
+  load_a<N, (N <= 6)>(am, a_of(a, b, MODE, D), a.lda);
 #pragma unroll kSlabUnroll
   for (int f = tid; f < F; f += NT)
   {
