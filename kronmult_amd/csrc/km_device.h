@@ -885,16 +885,124 @@ __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
     slab_stage<T, D, N, NT, EXACT, K + 1>(buf, a, b, tid);
 }
 
+// Pair layout of the slab accumulator: value r of a thread is element
+// 2*(tid + NT*(r/2)) + r%2 of the slab, so each thread owns consecutive
+// pairs and the run's Y preload and flush move 2 elements per lane and
+// instruction (half the instructions against the per-wave limit of
+// outstanding memory operations).  Used for owned runs that start from Y;
+// shared runs keep one element per lane and instruction, which keeps each
+// atomic instruction on contiguous addresses.
+template<typename T>
+using pair_t = T __attribute__((ext_vector_type(2)));
+
+// Pair load/store of Y; an output vector that is not aligned to a pair
+// (the caller's pointers need only element alignment) is accessed per
+// element instead.
+template<typename T>
+__device__ __forceinline__ pair_t<T> pair_ld(const T *p, bool al)
+{
+  if (al)
+    return *(gptr<const pair_t<T>>)(p);
+  return pair_t<T>{gld(p), gld(p + 1)};
+}
+template<typename T>
+__device__ __forceinline__ void pair_st(T *p, pair_t<T> v, bool al)
+{
+  if (al)
+    *(gptr<pair_t<T>>)(p) = v;
+  else
+  {
+    gst(p, v.x);
+    gst(p + 1, v.y);
+  }
+}
+
+template<int R, int NT, int S, typename T, typename C>
+__device__ __forceinline__ void pair_init(C (&acc)[R], const T *src, int tid,
+                                          bool preload)
+{
+  const bool al = reinterpret_cast<uintptr_t>(src) % sizeof(pair_t<T>) == 0;
+#pragma unroll
+  for (int rp = 0; rp < R / 2; rp++)
+  {
+    const int pr = tid + NT * rp;
+    pair_t<T> v = {T(0), T(0)};
+    if (preload && ((S / 2) % NT == 0 || pr < S / 2))
+      v = pair_ld(src + 2 * pr, al);
+    acc[2 * rp] = static_cast<C>(v.x);
+    acc[2 * rp + 1] = static_cast<C>(v.y);
+  }
+}
+
+template<int R, int NT, int S, typename T, typename C>
+__device__ __forceinline__ void pair_flush(T *dst, const C (&acc)[R], int tid,
+                                           int owned, bool preloaded)
+{
+  if (preloaded || owned)
+  {
+    const bool al = reinterpret_cast<uintptr_t>(dst) % sizeof(pair_t<T>) == 0;
+    pair_t<T> old[R / 2];
+    if (!preloaded)
+    {
+#pragma unroll
+      for (int rp = 0; rp < R / 2; rp++)
+      {
+        const int pr = tid + NT * rp;
+        old[rp] = {T(0), T(0)};
+        if ((S / 2) % NT == 0 || pr < S / 2)
+          old[rp] = pair_ld(dst + 2 * pr, al);
+      }
+    }
+#pragma unroll
+    for (int rp = 0; rp < R / 2; rp++)
+    {
+      const int pr = tid + NT * rp;
+      if ((S / 2) % NT == 0 || pr < S / 2)
+      {
+        pair_t<T> v = {static_cast<T>(acc[2 * rp]), static_cast<T>(acc[2 * rp + 1])};
+        if (!preloaded)
+          v = old[rp] + v;
+        pair_st(dst + 2 * pr, v, al);
+      }
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int rp = 0; rp < R / 2; rp++)
+    {
+      const int pr = tid + NT * rp;
+      if ((S / 2) % NT == 0 || pr < S / 2)
+      {
+        gatomic_add(dst + 2 * pr, static_cast<T>(acc[2 * rp]));
+        gatomic_add(dst + 2 * pr + 1, static_cast<T>(acc[2 * rp + 1]));
+      }
+    }
+  }
+}
+
 template<typename T, int D, int N, int NT, bool EXACT, bool PRELOAD, bool XNT>
 __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
                                           int p0, int p1)
 {
   constexpr int SL = cpow(N, D - 1);
-  constexpr int RS = (SL + NT - 1) / NT;
+  // measured: d6n6 plan 0.869 -> 0.771 ms; d5n8 plan 0.184 -> 0.186 ms
+  constexpr bool PAIR = PRELOAD && N <= 6 && SL % 2 == 0;
+  constexpr int RS = PAIR ? 2 * ((SL / 2 + NT - 1) / NT) : (SL + NT - 1) / NT;
   const int tid = threadIdx.x;
 
   using C = typename Acc<T>::type;
   C acc[N][RS];
+  auto flush_all = [&](T *dst) {
+#pragma unroll
+    for (int i1 = 0; i1 < N; i1++)
+    {
+      if constexpr (PAIR)
+        pair_flush<RS, NT, SL>(dst + i1 * SL, acc[i1], tid, a.owned, PRELOAD);
+      else
+        run_flush<RS, NT, SL, 2>(dst + i1 * SL, acc[i1], tid, a.owned, PRELOAD);
+    }
+  };
   T *target = nullptr;
   constexpr bool preload = PRELOAD;
 
@@ -942,14 +1050,15 @@ __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
       if (yb != target)
       {
         if (target)
-        {
-#pragma unroll
-          for (int i1 = 0; i1 < N; i1++)
-            run_flush<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned, preload);
-        }
+          flush_all(target);
 #pragma unroll
         for (int i1 = 0; i1 < N; i1++)
-          run_init<RS, NT, SL>(acc[i1], yb + i1 * SL, tid, preload);
+        {
+          if constexpr (PAIR)
+            pair_init<RS, NT, SL>(acc[i1], yb + i1 * SL, tid, preload);
+          else
+            run_init<RS, NT, SL>(acc[i1], yb + i1 * SL, tid, preload);
+        }
         target = yb;
       }
     }
@@ -968,22 +1077,42 @@ __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
 #pragma unroll
     for (int i1 = 0; i1 < N; i1++)
       a1[i1] = cld(A1 + (i1 + j1 * a.lda));
-#pragma unroll
-    for (int r = 0; r < RS; r++)
+    if constexpr (PAIR)
     {
-      int q = tid + NT * r;
-      if (SL % NT == 0 || q < SL)
-      {
-        C z = cur[q];
 #pragma unroll
-        for (int i1 = 0; i1 < N; i1++)
-          acc[i1][r] = madd<EXACT>(acc[i1][r], a1[i1], z);
+      for (int rp = 0; rp < RS / 2; rp++)
+      {
+        const int pr = tid + NT * rp;
+        if ((SL / 2) % NT == 0 || pr < SL / 2)
+        {
+          const pair_t<T> zz = *reinterpret_cast<const pair_t<T> *>(cur + 2 * pr);
+          const C z0 = zz.x, z1 = zz.y;
+#pragma unroll
+          for (int i1 = 0; i1 < N; i1++)
+          {
+            acc[i1][2 * rp] = madd<EXACT>(acc[i1][2 * rp], a1[i1], z0);
+            acc[i1][2 * rp + 1] = madd<EXACT>(acc[i1][2 * rp + 1], a1[i1], z1);
+          }
+        }
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int r = 0; r < RS; r++)
+      {
+        int q = tid + NT * r;
+        if (SL % NT == 0 || q < SL)
+        {
+          C z = cur[q];
+#pragma unroll
+          for (int i1 = 0; i1 < N; i1++)
+            acc[i1][r] = madd<EXACT>(acc[i1][r], a1[i1], z);
+        }
       }
     }
   }
-#pragma unroll
-  for (int i1 = 0; i1 < N; i1++)
-    run_flush<RS, NT, SL, 2>(target + i1 * SL, acc[i1], tid, a.owned, preload);
+  flush_all(target);
 }
 
 template<typename T, int D, int N, int NT, bool EXACT>

@@ -264,6 +264,35 @@ def test_plan_mismatch_rejected(km):
     plan.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("mode", ["atomic", "plan"])
+def test_outputs_aligned_to_one_element(km, dtype, mode):
+    """Output vectors only aligned to their element size (the reference's
+    pointers need no more): d = 6, n = 6 (kron_slab, whose owned runs move Y
+    two elements per lane and fall back to single elements here), 4:1."""
+    import torch
+    d, n, batch = 6, 6, 16
+    A, X, Y0 = rng_inputs(960, d, n, batch, batch // 4, dtype=dtype)
+    yidx = np.arange(batch) // 4
+    dev = torch.device("cuda:0")
+    s = n ** d
+    buf = torch.zeros(Y0.size + 1, dtype=torch.float64 if dtype == np.float64 else torch.float32, device=dev)
+    tY = buf[1:].view(batch // 4, s)
+    tY.copy_(torch.tensor(Y0, device=dev))
+    assert tY.data_ptr() % 16 != 0
+    pY = km.pointer_array(tY, yidx)
+    plan = km.Plan(d, n, X.itemsize, pY, batch) if mode == "plan" else None
+    tA = torch.tensor(A, device=dev)
+    tX = torch.tensor(X, device=dev)
+    km.kronmult_pbatched(d, n, tA, km.pointer_array(tX, np.arange(batch)), pY, None, batch,
+                         dtype="f64" if dtype == np.float64 else "f32", opt=km.options(plan=plan))
+    torch.cuda.synchronize()
+    want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
+    assert rel_err(tY.cpu().numpy(), want) <= (RTOL_FP64 if dtype == np.float64 else RTOL_FP32)
+    if plan is not None:
+        plan.close()
+
+
 @pytest.mark.parametrize("d,n,batch", [(6, 4, 2048), (5, 8, 256), (6, 5, 540), (3, 8, 16384)])
 @pytest.mark.parametrize("mode", ["atomic", "plan"])
 def test_streamed_inputs_vs_oracle(km, d, n, batch, mode):
