@@ -264,14 +264,16 @@ def test_plan_mismatch_rejected(km):
     plan.close()
 
 
+@pytest.mark.parametrize("d,n", [(6, 6), (6, 4), (4, 5)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("mode", ["atomic", "plan"])
-def test_outputs_aligned_to_one_element(km, dtype, mode):
+def test_outputs_aligned_to_one_element(km, d, n, dtype, mode):
     """Output vectors only aligned to their element size (the reference's
-    pointers need no more): d = 6, n = 6 (kron_slab, whose owned runs move Y
-    two elements per lane and fall back to single elements here), 4:1."""
+    pointers need no more), 4:1: kron_slab (owned runs move Y two elements
+    per lane and fall back to single elements here), kron_reg (wide stores
+    of consecutive elements), kron_lds."""
     import torch
-    d, n, batch = 6, 6, 16
+    batch = 16
     A, X, Y0 = rng_inputs(960, d, n, batch, batch // 4, dtype=dtype)
     yidx = np.arange(batch) // 4
     dev = torch.device("cuda:0")
