@@ -764,12 +764,17 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
         acc[r] = preload ? gld(yb + tid * RR + r) : T(0);
       target = yb;
     }
+    // the next entry's A is pulled into L2 before mode group PF: with three
+    // groups (d3n8) just before the last one, so that fewer of the lines
+    // are evicted before the next entry's scalar loads (plan 0.494 -> 0.474
+    // ms); with four or more, at the start (d4n8 measured slower later)
+    constexpr int PF = (NG == 3) ? NG - 1 : 0;
     T pf = 0;
-    if (kPrefetchA<N> && p + 1 < p1)
-      pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
 #pragma unroll
     for (int g = 0; g < NG; g++)
     {
+      if (g == PF && kPrefetchA<N> && p + 1 < p1)
+        pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
       reg_group<T, D, N, RM, EXACT, 0>(v, a, b, g);
       if (g + 1 < NG)
       {
