@@ -654,6 +654,7 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
       for (hipStream_t st : plan->streams)
         KM_HIP(hipStreamWaitEvent(st, plan->fork, 0));
     }
+    hipError_t launched = hipSuccess;
     for (size_t bi = 0; bi < plan->bins.size(); bi++)
     {
       const km_group_bin &bn = plan->bins[bi];
@@ -682,8 +683,12 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
         grid = (bn.cnt + bn.chunk - 1) / bn.chunk;
       }
       void *args[] = {&a};
-      KM_HIP(hipLaunchKernel(kb.fn, dim3(grid), dim3(kb.nt), args, 0, bs));
+      launched = hipLaunchKernel(kb.fn, dim3(grid), dim3(kb.nt), args, 0, bs);
+      if (launched != hipSuccess)
+        break;
     }
+    // the join also follows a failed launch, so that the caller's stream
+    // (or a graph being captured from it) is never left forked
     if (ns > 1)
     {
       for (int i = 0; i < ns; i++)
@@ -692,6 +697,7 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
         KM_HIP(hipStreamWaitEvent((hipStream_t)stream, plan->joins[i], 0));
       }
     }
+    KM_HIP(launched);
     return KM_SUCCESS;
   }
   KernelInfo k;

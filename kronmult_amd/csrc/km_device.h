@@ -6,15 +6,21 @@
 // column-major, A_k(i,j) = Ak[i + j*lda] (kgemm_nt.hpp:81).  Mode k (0-based)
 // of a tensor of n^d elements therefore has stride n^(d-1-k).
 //
-// Three kernel families, all without any global workspace of their own:
+// Kernel families:
 //   kron_lds   whole n^d tensor of the entry in LDS; every mode contraction is
 //              a pass of per-thread fibers (n loads, n*n FMAs, n stores);
+//   kron_reg   n a power of two: groups of modes contracted in registers, one
+//              LDS transpose between groups;
 //   kron_slab  n^d too large for LDS but n^(d-1) is not: the input is streamed
 //              in slabs X(j1,:) (direct-to-LDS loads, double buffered), modes
 //              2..d are contracted inside the slab and mode 1 is accumulated
 //              straight into per-thread registers;
+//   kron_sub + kron_pair  even one slab too large (d = 6, n = 7, 8): two
+//              passes through the caller's workspace;
+//   kron_grouped  entries of different n in one persistent launch;
 //   kron_global  any shape; stages through the caller's workspace in global
 //              memory (the reference's own storage scheme) -- the fallback.
+// The first three need no global workspace.
 //
 // Output handling shared by all families: a workgroup walks a contiguous range
 // of entries (in plan order when a plan is given).  Consecutive entries that
