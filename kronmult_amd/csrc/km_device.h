@@ -1527,6 +1527,82 @@ __global__ __launch_bounds__(NT) void kron_pair(EntryArgs<T> a, int e0, int e1)
     p1 = min(e1, p0 + a.chunk);
   }
   const bool live = (M % NT == 0) || (m < M);
+  if constexpr (!EXACT)
+  {
+    // Runs of consecutive entries with the same output are summed in
+    // registers and leave with one store (owned) or one atomic per element
+    // per run.  Mode 1 is applied as rank-1 updates, one row j1 of W at a
+    // time:  acc(i1, :) += A_1(i1, j1) * [A_2 W(j1, :)^T]^T,  so only the
+    // n x n accumulator and two n-value rows are live.
+    C acc[N][N];
+    T *target = nullptr;
+    auto flush = [&](T *yb) {
+      if (!live)
+        return;
+#pragma unroll
+      for (int i1 = 0; i1 < N; i1++)
+#pragma unroll
+        for (int i2 = 0; i2 < N; i2++)
+        {
+          T *yp = yb + (long long)(i1 * N + i2) * M + m;
+          if (a.owned)
+            gst(yp, static_cast<T>(acc[i1][i2]));
+          else
+            gatomic_add(yp, static_cast<T>(acc[i1][i2]));
+        }
+    };
+    for (int p = p0; p < p1; ++p)
+    {
+      const int b = entry_at(a, p);
+      T *yb = y_of(a, b);
+      if (yb != target)
+      {
+        if (target)
+          flush(target);
+        // an owned run starts from Y itself
+#pragma unroll
+        for (int i1 = 0; i1 < N; i1++)
+#pragma unroll
+          for (int i2 = 0; i2 < N; i2++)
+            acc[i1][i2] = (a.owned && live)
+                              ? static_cast<C>(gld(yb + (long long)(i1 * N + i2) * M + m))
+                              : C(0);
+        target = yb;
+      }
+      const T *wb = w_of(a, b, p - e0);
+      const T *A1 = a_of(a, b, 0, D);
+      const T *A2 = a_of(a, b, 1, D);
+#pragma unroll 1
+      for (int j1 = 0; j1 < N; j1++)
+      {
+        C w[N];
+#pragma unroll
+        for (int j2 = 0; j2 < N; j2++)
+          w[j2] = live ? static_cast<C>(gld(wb + (long long)(j1 * N + j2) * M + m)) : C(0);
+        C u[N];
+#pragma unroll
+        for (int i2 = 0; i2 < N; i2++)
+        {
+          C t = 0;
+#pragma unroll
+          for (int j2 = 0; j2 < N; j2++)
+            t = madd<false>(t, static_cast<C>(cld(A2 + (i2 + j2 * a.lda))), w[j2]);
+          u[i2] = static_cast<C>(static_cast<T>(t));
+        }
+#pragma unroll
+        for (int i1 = 0; i1 < N; i1++)
+        {
+          const C a1 = static_cast<C>(cld(A1 + (i1 + j1 * a.lda)));
+#pragma unroll
+          for (int i2 = 0; i2 < N; i2++)
+            acc[i1][i2] = madd<false>(acc[i1][i2], a1, u[i2]);
+        }
+      }
+    }
+    if (target)
+      flush(target);
+    return;
+  }
   for (int p = p0; p < p1; ++p)
   {
     const int b = entry_at(a, p);
