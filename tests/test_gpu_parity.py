@@ -435,6 +435,28 @@ def test_twopass_vbatched_square_chunked(km):
     assert rel_err(got, want) <= RTOL_FP64
 
 
+@pytest.mark.parametrize("mode", ["atomic", "plan"])
+def test_twopass_merged_runs_integer_bitwise(km, mode):
+    """Pass 2 sums each output run in registers (runs of 3 entries here, d=6,
+    n=7): with small-integer inputs every sum is exact, so a dropped or
+    doubly counted entry, or a run flushed twice, shows bitwise."""
+    import torch
+    d, n, batch = 6, 7, 12
+    A, X, Y0 = int_inputs(81, d, n, batch, 4)
+    yidx = np.arange(batch) // 3
+    opt = None
+    plan = None
+    if mode == "plan":
+        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
+        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
+        opt = km.options(plan=plan)
+    got = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
+    want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
+    assert np.array_equal(got, want)
+    if plan is not None:
+        plan.close()
+
+
 def test_twopass_plan_deterministic(km):
     import torch
     d, n, batch = 6, 8, 6
