@@ -233,6 +233,8 @@ bool pick_onchip(int d, int n, const Resolved &r, KernelInfo *k)
     return true;
   if (!fwd && km::slab_kernel<T>(d, n, r.exact, k))
     return true;
+  if (fwd && km::slab_fwd_kernel<T>(d, n, k))
+    return true;
   return false;
 }
 

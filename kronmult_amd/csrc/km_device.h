@@ -871,15 +871,16 @@ __device__ __forceinline__ void slab_async_load(T *dst, const T *src, int tid)
   }
 }
 
-template<typename T, int D, int N, int NT, bool EXACT, int K>
+template<typename T, int D, int N, int NT, bool EXACT, int K, bool FWD = false>
 __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
                                            int tid)
 {
-  // K runs 0..D-2 over in-slab modes D-1 .. 1 (global mode numbering)
+  // K runs 0..D-2 over the in-slab modes (global mode numbering): D-1 .. 1,
+  // or 1 .. D-1 with FWD
   using C = typename Acc<T>::type;
   constexpr int SL = cpow(N, D - 1);
   constexpr int F = SL / N;
-  constexpr int MODE = D - 1 - K;
+  constexpr int MODE = FWD ? 1 + K : D - 1 - K;
   constexpr int STRIDE = cpow(N, D - 1 - MODE);
   C am[N][N];
   // float: widened A back in SGPRs where n*n doubles fit there (n <= 6)
@@ -893,7 +894,7 @@ __device__ __forceinline__ void slab_stage(T *buf, const EntryArgs<T> &a, int b,
   }
   lds_barrier();
   if constexpr (K + 2 < D)
-    slab_stage<T, D, N, NT, EXACT, K + 1>(buf, a, b, tid);
+    slab_stage<T, D, N, NT, EXACT, K + 1, FWD>(buf, a, b, tid);
 }
 
 // Pair layout of the slab accumulator: value r of a thread is element
@@ -1147,6 +1148,117 @@ __global__ __launch_bounds__(NT) void kron_slab(EntryArgs<T> a)
     slab_body<T, D, N, NT, EXACT, false, true>(a, sb0, sb1, p0, p1);
   else
     slab_body<T, D, N, NT, EXACT, false, false>(a, sb0, sb1, p0, p1);
+}
+
+// ---------------------------------------------------------------------------
+// kron_slab_fwd: the forward mode order 1..d with the reference's rounding
+// (KM_FLAG_EXACT with KM_VARIANT_FORWARD, the reference's default recursive
+// build).  Mode 1 comes first here, so the slabs X(j1, :) are streamed into
+// a register accumulator of the mode-1 result Z(i1, :) = sum_j1 A_1(i1, j1)
+// X(j1, :) (j1 ascending, separate multiply and add); then each output slab
+// Z(i1, :) goes to LDS, modes 2..d are contracted there in order, and the
+// slab is added to Y(i1, :).  Same passes and bytes as kron_slab; outputs
+// are updated once per entry (no run merging: the registers hold Z).
+// ---------------------------------------------------------------------------
+template<typename T, int D, int N, int NT>
+__device__ __forceinline__ void slab_fwd_body(const EntryArgs<T> &a, T *sb0,
+                                              T *sb1, int p0, int p1)
+{
+  constexpr int SL = cpow(N, D - 1);
+  constexpr int RS = (SL + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  using C = typename Acc<T>::type;
+  C acc[N][RS];
+
+  const int nslab = (p1 - p0) * N;
+  slab_async_load<T, SL, NT, false>(sb0, x_of(a, entry_at(a, p0)), tid);
+  for (int t = 0; t < nslab; ++t)
+  {
+    const int p = p0 + t / N;
+    const int j1 = t - (t / N) * N;
+    const int b = entry_at(a, p);
+    T *cur = (t & 1) ? sb1 : sb0;
+    T *nxt = (t & 1) ? sb0 : sb1;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (t + 1 < nslab)
+    {
+      const int pn = p0 + (t + 1) / N;
+      const int jn = (t + 1) - ((t + 1) / N) * N;
+      slab_async_load<T, SL, NT, false>(nxt, x_of(a, entry_at(a, pn)) + (size_t)jn * SL,
+                                        tid);
+    }
+    const T *A1 = a_of(a, b, 0, D);
+    C a1[N];
+#pragma unroll
+    for (int i1 = 0; i1 < N; i1++)
+      a1[i1] = cld(A1 + (i1 + j1 * a.lda));
+#pragma unroll
+    for (int r = 0; r < RS; r++)
+    {
+      const int q = tid + NT * r;
+      if (SL % NT == 0 || q < SL)
+      {
+        const C x = cur[q];
+#pragma unroll
+        for (int i1 = 0; i1 < N; i1++)
+          acc[i1][r] = madd<true>(j1 == 0 ? C(0) : acc[i1][r], a1[i1], x);
+      }
+    }
+    if (j1 == N - 1)
+    {
+      // the entry's output slabs, one at a time, in the buffer of its last
+      // input slab (the next DMA goes to the other buffer)
+      T *yb = y_of(a, b);
+#pragma unroll
+      for (int i1 = 0; i1 < N; i1++)
+      {
+        lds_barrier(); // readers of cur are done
+#pragma unroll
+        for (int r = 0; r < RS; r++)
+        {
+          const int q = tid + NT * r;
+          if (SL % NT == 0 || q < SL)
+            cur[q] = static_cast<T>(acc[i1][r]);
+        }
+        lds_barrier();
+        slab_stage<T, D, N, NT, true, 0, true>(cur, a, b, tid);
+#pragma unroll
+        for (int r = 0; r < RS; r++)
+        {
+          const int q = tid + NT * r;
+          if (SL % NT == 0 || q < SL)
+          {
+            T *yp = yb + (size_t)i1 * SL + q;
+            if (a.owned)
+            {
+              // the previous entry of this output was written by this thread;
+              // a device-coherent load does not see a stale cached line
+              const T y = __hip_atomic_load((gptr<T>)(yp), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+              gst(yp, y + cur[q]);
+            }
+            else
+              gatomic_add(yp, cur[q]);
+          }
+        }
+      }
+    }
+  }
+}
+
+template<typename T, int D, int N, int NT>
+__global__ __launch_bounds__(NT) void kron_slab_fwd(EntryArgs<T> a)
+{
+  constexpr int SL = cpow(N, D - 1);
+  static_assert((SL * sizeof(T)) % 16 == 0, "slab must be whole 16-byte chunks");
+  __shared__ __attribute__((aligned(16))) T sb0[SL];
+  __shared__ __attribute__((aligned(16))) T sb1[SL];
+  int p0, p1;
+  wg_range(a, p0, p1);
+  if (p0 >= p1)
+    return;
+  slab_fwd_body<T, D, N, NT>(a, sb0, sb1, p0, p1);
 }
 
 // ---------------------------------------------------------------------------

@@ -93,6 +93,10 @@ bool reg_kernel(int d, int n, bool exact, KernelInfo *out);
 template<typename T>
 bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
 
+// kron_slab_fwd (exact, forward mode order) for (d, n) if one was built
+template<typename T>
+bool slab_fwd_kernel(int d, int n, KernelInfo *out);
+
 // kron_sub + kron_pair for (d, n) if built
 template<typename T>
 bool twopass_kernels(int d, int n, bool exact, KernelInfo *sub,

@@ -73,6 +73,41 @@ def test_exact_mode_bitwise(km, d, n, variant):
     assert np.array_equal(got, want), (d, n, variant, rel_err(got, want))
 
 
+@pytest.mark.parametrize("d,n", [(6, 6), (5, 8)])
+@pytest.mark.parametrize("mode", ["atomic", "plan"])
+def test_exact_forward_slab(km, d, n, mode):
+    """EXACT with the forward order above the LDS size runs kron_slab_fwd
+    (mode 1 accumulated in registers first).  With a plan every output is
+    owned and updated once per entry in entry order, which is the oracle's
+    sequential order: bitwise even for shared outputs (three entries each,
+    interleaved).  Atomic: single-entry outputs bitwise."""
+    import torch
+    opt0 = km.options(exact=True, variant=km.VARIANT_FORWARD)
+    assert km.query_kernel(d, n, 8, opt0)[0] == km.FAMILY_SLAB
+    batch = 6
+    A, X, Y0 = rng_inputs(820 + d * n, d, n, batch, batch)
+    if mode == "plan":
+        yidx = np.array([0, 0, 1, 1, 0, 1])
+        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
+        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
+        opt = km.options(exact=True, variant=km.VARIANT_FORWARD, plan=plan)
+    else:
+        yidx = np.arange(batch)
+        plan = None
+        opt = opt0
+    got = gpu_pbatched(A, X, Y0, yidx, opt=opt)
+    # one oracle thread: entries reach a shared output in entry order
+    threads = O.num_threads()
+    O.set_num_threads(1)
+    try:
+        want = O.pbatched(A, X, Y0, yidx, variant=O.RECURSIVE)
+    finally:
+        O.set_num_threads(threads)
+    assert np.array_equal(got, want), rel_err(got, want)
+    if plan is not None:
+        plan.close()
+
+
 @pytest.mark.parametrize("d,n", [(2, 3), (3, 4), (4, 5), (6, 3), (6, 4), (6, 6)])
 def test_exact_mode_forced_global_bitwise(km, d, n):
     batch = 4
