@@ -219,14 +219,17 @@ struct Acc<float>
 };
 
 // A wave-uniform double held in VGPRs (a float widened by the vector unit)
-// moved to an SGPR pair.  The compiler folds its own readfirstlane of a value
-// it knows to be uniform and then keeps the double in two VGPRs per lane.
+// moved to an SGPR pair with the readfirstlane builtin.  The compiler folds
+// a readfirstlane of a value it knows to be uniform (and then keeps the
+// double in two VGPRs per lane), so the bits are first OR-ed with mbcnt(0),
+// which is 0 in every lane but not known to be uniform.  The builtin, unlike
+// inline asm, gets the wait states the result of the widening needs.
 __device__ __forceinline__ double to_sgpr(double v)
 {
+  const unsigned z = __builtin_amdgcn_mbcnt_lo(0u, 0u);
   const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  unsigned lo, hi;
-  asm volatile("v_readfirstlane_b32 %0, %1" : "=s"(lo) : "v"((unsigned)u));
-  asm volatile("v_readfirstlane_b32 %0, %1" : "=s"(hi) : "v"((unsigned)(u >> 32)));
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u | z);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32) | z);
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 

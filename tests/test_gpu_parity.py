@@ -73,6 +73,26 @@ def test_exact_mode_bitwise(km, d, n, variant):
     assert np.array_equal(got, want), (d, n, variant, rel_err(got, want))
 
 
+@pytest.mark.parametrize("d,n", [(2, 5), (4, 6), (3, 8), (6, 4), (6, 6), (5, 8)])
+@pytest.mark.parametrize("variant", ["backward", "forward"])
+def test_exact_mode_bitwise_float(km, d, n, variant):
+    """Float data (every stage's result rounded to float, dot products in
+    double, as the reference's float templates): EXACT is bitwise to the
+    oracle for single-entry outputs in kron_lds, kron_reg, kron_slab and
+    kron_slab_fwd.  kron_slab at n <= 6 moves the widened A to SGPRs; a stale
+    low half of those doubles would show here as a 1e-6 error."""
+    if variant == "backward":
+        opt, ovar = km.options(exact=True, variant=km.VARIANT_BACKWARD), O.BACKWARD
+    else:
+        opt, ovar = km.options(exact=True, variant=km.VARIANT_FORWARD), O.RECURSIVE
+    batch = 4
+    A, X, Y0 = rng_inputs(830 + d * n, d, n, batch, batch, dtype=np.float32)
+    yidx = np.arange(batch)
+    got = gpu_pbatched(A, X, Y0, yidx, opt=opt)
+    want = O.pbatched(A, X, Y0, yidx, variant=ovar)
+    assert np.array_equal(got, want), rel_err(got, want)
+
+
 @pytest.mark.parametrize("d,n", [(6, 6), (5, 8)])
 @pytest.mark.parametrize("mode", ["atomic", "plan"])
 def test_exact_forward_slab(km, d, n, mode):
