@@ -338,6 +338,74 @@ km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
   return KM_SUCCESS;
 }
 
+// Forward two-pass family (exact, mode order 1..d): pass 1 over the chunk's
+// entries and the positions of the trailing modes, pass 2 over the chunk's
+// entries and the n*n sub-slabs.
+template<typename T>
+km_status launch_twopass_fwd(Call<T> &c, const Resolved &r, const KernelInfo &k1,
+                             const KernelInfo &k2, hipStream_t stream)
+{
+  const int batch = c.a.batch;
+  const int d = c.sh.d, n = c.sh.n[0];
+  const int nn = n * n;
+  const long long M = ipow(n, d - 2);
+  const int cus = device_cus();
+  if (cus <= 0)
+    return KM_ERROR_HIP;
+  const long long chunk_entries = c.a.slot_by_entry ? batch : c.slots;
+  if (chunk_entries < 1)
+    return KM_ERROR_INSUFFICIENT_WORKSPACE;
+  if (r.plan)
+  {
+    const km_plan p = r.plan;
+    if (p->batch != batch || p->elem_bytes != (int)sizeof(T) || p->d != d ||
+        p->n != n)
+      return KM_ERROR_PLAN_MISMATCH;
+    if (chunk_entries < batch)
+      return KM_ERROR_NOT_SUPPORTED; // a plan needs every entry's workspace
+    c.a.order = p->order;
+  }
+  const int occ1 = occupancy(k1.fn, k1.nt);
+  const int occ2 = occupancy(k2.fn, k2.nt);
+  const int gy1 = (int)((M + k1.nt - 1) / k1.nt);
+  for (long long e0 = 0; e0 < batch; e0 += chunk_entries)
+  {
+    const int cnt = (int)std::min<long long>(chunk_entries, batch - e0);
+    int e0i = (int)e0;
+    int e1i = (int)(e0 + cnt);
+    // pass 1: entries x positions
+    {
+      long long want = std::max(1LL, (long long)cus * occ1 * 2 / gy1);
+      long long g = std::min<long long>(want, cnt);
+      c.a.chunk = (int)((cnt + g - 1) / g);
+      c.a.ranges = nullptr;
+      int gx = (cnt + c.a.chunk - 1) / c.a.chunk;
+      void *args1[] = {&c.a, &e0i, &e1i};
+      KM_HIP(hipLaunchKernel(k1.fn, dim3(gx, gy1), dim3(k1.nt), args1, 0, stream));
+    }
+    // pass 2: entries (plan ranges, or chunks) x sub-slabs
+    int gx;
+    if (r.plan)
+    {
+      c.a.ranges = r.plan->ranges;
+      c.a.owned = r.plan->owned;
+      gx = r.plan->ngroups;
+    }
+    else
+    {
+      long long want = std::max(1LL, (long long)cus * occ2 * 2 / nn);
+      long long g = std::min<long long>(want, cnt);
+      c.a.chunk = (int)((cnt + g - 1) / g);
+      c.a.ranges = nullptr;
+      c.a.owned = 0;
+      gx = (cnt + c.a.chunk - 1) / c.a.chunk;
+    }
+    void *args2[] = {&c.a, &e0i, &e1i};
+    KM_HIP(hipLaunchKernel(k2.fn, dim3(gx, nn), dim3(k2.nt), args2, 0, stream));
+  }
+  return KM_SUCCESS;
+}
+
 template<typename T>
 km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
 {
@@ -354,6 +422,16 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
       if (c.slots < 1)
         return KM_ERROR_INSUFFICIENT_WORKSPACE;
       return launch_twopass(c, r, k1, k2, stream);
+    }
+  }
+  if (!onchip && c.square && !r.force_global && r.fwd && r.exact)
+  {
+    KernelInfo k1, k2;
+    if (km::twopass_fwd_kernels<T>(c.sh.d, c.sh.n[0], &k1, &k2))
+    {
+      if (c.slots < 1)
+        return KM_ERROR_INSUFFICIENT_WORKSPACE;
+      return launch_twopass_fwd(c, r, k1, k2, stream);
     }
   }
   if (!onchip)
@@ -1086,6 +1164,9 @@ extern "C"
       twopass = (elem_bytes == 8)
                     ? km::twopass_kernels<double>(d, n, r.exact, &k, &k2)
                     : km::twopass_kernels<float>(d, n, r.exact, &k, &k2);
+    else if (!onchip && !r.force_global)
+      twopass = (elem_bytes == 8) ? km::twopass_fwd_kernels<double>(d, n, &k, &k2)
+                                  : km::twopass_fwd_kernels<float>(d, n, &k, &k2);
     if (!onchip && !twopass)
       k = (elem_bytes == 8) ? km::global_kernel<double>(r.exact)
                             : km::global_kernel<float>(r.exact);

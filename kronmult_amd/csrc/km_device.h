@@ -1815,6 +1815,168 @@ __global__ __launch_bounds__(NT) void kron_pair(EntryArgs<T> a, int e0, int e1)
 }
 
 // ---------------------------------------------------------------------------
+// Forward two-pass family (KM_FLAG_EXACT with KM_VARIANT_FORWARD, where one
+// slab overflows LDS): the mirror of kron_sub + kron_pair for the mode order
+// 1..d.
+//   pass 1 (kron_pair_fwd): per position m of the trailing modes, modes 1
+//     then 2 of the n x n values X(:, :, m) in registers, each result
+//     rounded to T as a stage output, into W(i1, i2, m);
+//   pass 2 (kron_sub_fwd): per (i1, i2) sub-slab W(i1, i2, :) in LDS, modes
+//     3..d in order, then added to Y(i1, i2, :) -- atomically, or with a plan
+//     as a read-modify-write per entry in entry order (blockIdx.y fixes the
+//     sub-slab, so one workgroup updates each part of an owned output).
+// ---------------------------------------------------------------------------
+template<typename T, int D, int N, int NT, int MODE>
+__device__ __forceinline__ void forward_stages(T *buf, const EntryArgs<T> &a,
+                                               int b, int tid)
+{
+  constexpr int SZ = cpow(N, D - 2);
+  constexpr int F = SZ / N;
+  constexpr int STRIDE = cpow(N, D - 1 - MODE);
+  stage_apply<T, N, NT, true, F, STRIDE>(buf, a_of(a, b, MODE, D), a.lda, tid);
+  lds_barrier();
+  if constexpr (MODE + 1 < D)
+    forward_stages<T, D, N, NT, MODE + 1>(buf, a, b, tid);
+}
+
+template<typename T, int D, int N, int NT>
+__global__ __launch_bounds__(NT) void kron_pair_fwd(EntryArgs<T> a, int e0, int e1)
+{
+  using C = typename Acc<T>::type;
+  constexpr int M = cpow(N, D - 2);
+  const int tid = threadIdx.x;
+  const int m = blockIdx.y * NT + tid;
+  const int p0 = e0 + blockIdx.x * a.chunk;
+  const int p1 = min(e1, p0 + a.chunk);
+  if (!((M % NT == 0) || (m < M)))
+    return;
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    const T *xb = x_of(a, b);
+    T *wb = w_of(a, b, p - e0);
+    C x[N][N];
+#pragma unroll
+    for (int j1 = 0; j1 < N; j1++)
+#pragma unroll
+      for (int j2 = 0; j2 < N; j2++)
+        x[j1][j2] = static_cast<C>(gld(xb + (long long)(j1 * N + j2) * M + m));
+    // mode 1: x(i1, j2) <- sum_j1 A_1(i1, j1) x(j1, j2), one column at a time
+    const T *A1 = a_of(a, b, 0, D);
+#pragma unroll
+    for (int j2 = 0; j2 < N; j2++)
+    {
+      C t[N];
+#pragma unroll
+      for (int i1 = 0; i1 < N; i1++)
+      {
+        C acc = 0;
+#pragma unroll
+        for (int j1 = 0; j1 < N; j1++)
+          acc = madd<true>(acc, static_cast<C>(cld(A1 + (i1 + j1 * a.lda))), x[j1][j2]);
+        t[i1] = static_cast<C>(static_cast<T>(acc));
+      }
+#pragma unroll
+      for (int i1 = 0; i1 < N; i1++)
+        x[i1][j2] = t[i1];
+    }
+    // mode 2: W(i1, i2) = sum_j2 A_2(i2, j2) x(i1, j2), one row at a time
+    const T *A2 = a_of(a, b, 1, D);
+#pragma unroll
+    for (int i1 = 0; i1 < N; i1++)
+    {
+#pragma unroll
+      for (int i2 = 0; i2 < N; i2++)
+      {
+        C acc = 0;
+#pragma unroll
+        for (int j2 = 0; j2 < N; j2++)
+          acc = madd<true>(acc, static_cast<C>(cld(A2 + (i2 + j2 * a.lda))), x[i1][j2]);
+        gst(wb + (long long)(i1 * N + i2) * M + m, static_cast<T>(acc));
+      }
+    }
+  }
+}
+
+template<typename T, int D, int N, int NT>
+__global__ __launch_bounds__(NT) void kron_sub_fwd(EntryArgs<T> a, int e0, int e1)
+{
+  constexpr int M = cpow(N, D - 2);
+  constexpr int R = (M + NT - 1) / NT;
+  __shared__ T buf[M];
+  const int tid = threadIdx.x;
+  const int js = blockIdx.y; // sub-slab (i1, i2) = (js / N, js % N)
+  int p0, p1;
+  if (a.ranges)
+  {
+    p0 = cld(a.ranges + blockIdx.x);
+    p1 = cld(a.ranges + blockIdx.x + 1);
+  }
+  else
+  {
+    p0 = e0 + blockIdx.x * a.chunk;
+    p1 = min(e1, p0 + a.chunk);
+  }
+  if (p0 >= p1)
+    return;
+  T wr[R];
+  {
+    const T *wp = w_of(a, entry_at(a, p0), p0 - e0) + (long long)js * M;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (M % NT == 0 || q < M)
+        wr[r] = gld(wp + q);
+    }
+  }
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    lds_barrier(); // buf free
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (M % NT == 0 || q < M)
+        buf[q] = wr[r];
+    }
+    if (p + 1 < p1)
+    {
+      const T *wp = w_of(a, entry_at(a, p + 1), p + 1 - e0) + (long long)js * M;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+      {
+        const int q = tid + NT * r;
+        if (M % NT == 0 || q < M)
+          wr[r] = gld(wp + q);
+      }
+    }
+    lds_barrier();
+    forward_stages<T, D, N, NT, 2>(buf, a, b, tid);
+    T *yb = y_of(a, b) + (long long)js * M;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NT * r;
+      if (M % NT == 0 || q < M)
+      {
+        if (a.owned)
+        {
+          // the previous entry of this output was written by this thread;
+          // a device-coherent load does not see a stale cached line
+          const T y = __hip_atomic_load((gptr<T>)(yb + q), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+          gst(yb + q, y + buf[q]);
+        }
+        else
+          gatomic_add(yb + q, buf[q]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // kron_global: any d <= 6 and any (rectangular) m_k x n_k with
 // n_k <= KM_GLOBAL_MAX_N; one workgroup per entry; stages go through the caller's workspace as in the reference
 // (kronm_backward.hpp:92-139).  Square shapes update one buffer in place;

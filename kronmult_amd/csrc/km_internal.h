@@ -97,6 +97,10 @@ bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
 template<typename T>
 bool slab_fwd_kernel(int d, int n, KernelInfo *out);
 
+// kron_pair_fwd + kron_sub_fwd (exact, forward mode order) for (d, n) if built
+template<typename T>
+bool twopass_fwd_kernels(int d, int n, KernelInfo *pair, KernelInfo *sub);
+
 // kron_sub + kron_pair for (d, n) if built
 template<typename T>
 bool twopass_kernels(int d, int n, bool exact, KernelInfo *sub,

@@ -128,6 +128,40 @@ def test_exact_forward_slab(km, d, n, mode):
         plan.close()
 
 
+@pytest.mark.parametrize("d,n", [(6, 7), (6, 8)])
+@pytest.mark.parametrize("mode", ["atomic", "plan"])
+def test_exact_forward_twopass(km, d, n, mode):
+    """EXACT with the forward order where one slab overflows LDS runs the
+    forward two-pass pair (modes 1, 2 per position, then modes 3..d per
+    sub-slab).  With a plan each part of an owned output is updated by one
+    workgroup in entry order: bitwise to a single-thread oracle even for
+    shared outputs.  Atomic: single-entry outputs bitwise."""
+    import torch
+    opt0 = km.options(exact=True, variant=km.VARIANT_FORWARD)
+    assert km.query_kernel(d, n, 8, opt0)[0] == km.FAMILY_TWOPASS
+    batch = 6 if n == 7 else 4
+    A, X, Y0 = rng_inputs(840 + d * n, d, n, batch, batch)
+    if mode == "plan":
+        yidx = np.array([0, 0, 1, 1, 0, 1])[:batch]
+        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
+        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
+        opt = km.options(exact=True, variant=km.VARIANT_FORWARD, plan=plan)
+    else:
+        yidx = np.arange(batch)
+        plan = None
+        opt = opt0
+    got = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
+    threads = O.num_threads()
+    O.set_num_threads(1)
+    try:
+        want = O.pbatched(A, X, Y0, yidx, variant=O.RECURSIVE)
+    finally:
+        O.set_num_threads(threads)
+    assert np.array_equal(got, want), rel_err(got, want)
+    if plan is not None:
+        plan.close()
+
+
 @pytest.mark.parametrize("d,n", [(2, 3), (3, 4), (4, 5), (6, 3), (6, 4), (6, 6)])
 def test_exact_mode_forced_global_bitwise(km, d, n):
     batch = 4
