@@ -1642,176 +1642,89 @@ __global__ __launch_bounds__(NT) void kron_pair(EntryArgs<T> a, int e0, int e1)
     p1 = min(e1, p0 + a.chunk);
   }
   const bool live = (M % NT == 0) || (m < M);
-  if constexpr (!EXACT)
-  {
-    // Runs of consecutive entries with the same output are summed in
-    // registers and leave with one store (owned) or one atomic per element
-    // per run.  Mode 1 is applied as rank-1 updates, one row j1 of W at a
-    // time:  acc(i1, :) += A_1(i1, j1) * [A_2 W(j1, :)^T]^T,  so only the
-    // n x n accumulator and two n-value rows are live.
-    C acc[N][N];
-    T *target = nullptr;
-    auto flush = [&](T *yb) {
-      if (!live)
-        return;
+  // Mode 1 is applied as rank-1 updates, one row j1 of W at a time:
+  //   acc(i1, :) += A_1(i1, j1) * [A_2 W(j1, :)^T]^T,
+  // so only the n x n accumulator and two n-value rows are live (each acc
+  // element still sums its terms over j1 ascending).  FMA calls sum runs of
+  // consecutive entries with the same output in registers and leave with
+  // one store (owned, started from Y) or one atomic per element per run;
+  // EXACT keeps the reference's Y + (entry sum), one update per entry.
+  C acc[N][N];
+  T *target = nullptr;
+  auto flush = [&](T *yb) {
+    if (!live)
+      return;
 #pragma unroll
-      for (int i1 = 0; i1 < N; i1++)
+    for (int i1 = 0; i1 < N; i1++)
 #pragma unroll
-        for (int i2 = 0; i2 < N; i2++)
-        {
-          T *yp = yb + (long long)(i1 * N + i2) * M + m;
-          if (a.owned)
-            gst(yp, static_cast<T>(acc[i1][i2]));
-          else
-            gatomic_add(yp, static_cast<T>(acc[i1][i2]));
-        }
-    };
-    for (int p = p0; p < p1; ++p)
-    {
-      const int b = entry_at(a, p);
-      T *yb = y_of(a, b);
-      if (yb != target)
+      for (int i2 = 0; i2 < N; i2++)
       {
-        if (target)
-          flush(target);
-        // an owned run starts from Y itself
-#pragma unroll
-        for (int i1 = 0; i1 < N; i1++)
-#pragma unroll
-          for (int i2 = 0; i2 < N; i2++)
-            acc[i1][i2] = (a.owned && live)
-                              ? static_cast<C>(gld(yb + (long long)(i1 * N + i2) * M + m))
-                              : C(0);
-        target = yb;
-      }
-      const T *wb = w_of(a, b, p - e0);
-      const T *A1 = a_of(a, b, 0, D);
-      const T *A2 = a_of(a, b, 1, D);
-#pragma unroll 1
-      for (int j1 = 0; j1 < N; j1++)
-      {
-        C w[N];
-#pragma unroll
-        for (int j2 = 0; j2 < N; j2++)
-          w[j2] = live ? static_cast<C>(gld(wb + (long long)(j1 * N + j2) * M + m)) : C(0);
-        C u[N];
-#pragma unroll
-        for (int i2 = 0; i2 < N; i2++)
+        T *yp = yb + (long long)(i1 * N + i2) * M + m;
+        const T v = static_cast<T>(acc[i1][i2]);
+        if (!a.owned)
+          gatomic_add(yp, v);
+        else if constexpr (!EXACT)
+          gst(yp, v);
+        else
         {
-          C t = 0;
-#pragma unroll
-          for (int j2 = 0; j2 < N; j2++)
-            t = madd<false>(t, static_cast<C>(cld(A2 + (i2 + j2 * a.lda))), w[j2]);
-          u[i2] = static_cast<C>(static_cast<T>(t));
-        }
-#pragma unroll
-        for (int i1 = 0; i1 < N; i1++)
-        {
-          const C a1 = static_cast<C>(cld(A1 + (i1 + j1 * a.lda)));
-#pragma unroll
-          for (int i2 = 0; i2 < N; i2++)
-            acc[i1][i2] = madd<false>(acc[i1][i2], a1, u[i2]);
+          // the previous entry of this output was written by this thread;
+          // a device-coherent load does not see a stale cached line
+          const T y = __hip_atomic_load((gptr<T>)(yp), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+          gst(yp, y + v);
         }
       }
-    }
-    if (target)
-      flush(target);
-    return;
-  }
+  };
   for (int p = p0; p < p1; ++p)
   {
     const int b = entry_at(a, p);
-    const T *wb = w_of(a, b, p - e0);
-    C z[N][N];
-    if (live)
+    T *yb = y_of(a, b);
+    if (EXACT || yb != target)
     {
+      if (target)
+        flush(target);
+      // an owned FMA run starts from Y itself
 #pragma unroll
-      for (int j1 = 0; j1 < N; j1++)
-#pragma unroll
-        for (int j2 = 0; j2 < N; j2++)
-          z[j1][j2] = static_cast<C>(gld(wb + (long long)(j1 * N + j2) * M + m));
-    }
-    constexpr int RB = (N <= 6) ? N : (N + 1) / 2;
-    {
-      // mode 2: u(j1, i2) = sum_j2 A_2(i2, j2) z(j1, j2)
-      const T *A2 = a_of(a, b, 1, D);
-      C u[N][N];
-#pragma unroll
-      for (int r0 = 0; r0 < N; r0 += RB)
-      {
-        C am[RB][N];
-#pragma unroll
-        for (int j = 0; j < N; j++)
-#pragma unroll
-          for (int i = 0; i < RB; i++)
-            if (r0 + i < N)
-              am[i][j] = static_cast<C>(cld(A2 + (r0 + i + j * a.lda)));
-#pragma unroll
-        for (int j1 = 0; j1 < N; j1++)
-#pragma unroll
-          for (int i = 0; i < RB; i++)
-            if (r0 + i < N)
-            {
-              C acc = 0;
-#pragma unroll
-              for (int j2 = 0; j2 < N; j2++)
-                acc = madd<EXACT>(acc, am[i][j2], z[j1][j2]);
-              u[j1][r0 + i] = static_cast<C>(static_cast<T>(acc));
-            }
-      }
-      // mode 1: z(i1, i2) = sum_j1 A_1(i1, j1) u(j1, i2)
-      const T *A1 = a_of(a, b, 0, D);
-#pragma unroll
-      for (int r0 = 0; r0 < N; r0 += RB)
-      {
-        C am[RB][N];
-#pragma unroll
-        for (int j = 0; j < N; j++)
-#pragma unroll
-          for (int i = 0; i < RB; i++)
-            if (r0 + i < N)
-              am[i][j] = static_cast<C>(cld(A1 + (r0 + i + j * a.lda)));
+      for (int i1 = 0; i1 < N; i1++)
 #pragma unroll
         for (int i2 = 0; i2 < N; i2++)
-#pragma unroll
-          for (int i = 0; i < RB; i++)
-            if (r0 + i < N)
-            {
-              C acc = 0;
-#pragma unroll
-              for (int j1 = 0; j1 < N; j1++)
-                acc = madd<EXACT>(acc, am[i][j1], u[j1][i2]);
-              z[r0 + i][i2] = acc;
-            }
-      }
+          acc[i1][i2] = (!EXACT && a.owned && live)
+                            ? static_cast<C>(gld(yb + (long long)(i1 * N + i2) * M + m))
+                            : C(0);
+      target = yb;
     }
-    if (live)
+    const T *wb = w_of(a, b, p - e0);
+    const T *A1 = a_of(a, b, 0, D);
+    const T *A2 = a_of(a, b, 1, D);
+#pragma unroll 1
+    for (int j1 = 0; j1 < N; j1++)
     {
-      T *yb = y_of(a, b);
+      C w[N];
+#pragma unroll
+      for (int j2 = 0; j2 < N; j2++)
+        w[j2] = live ? static_cast<C>(gld(wb + (long long)(j1 * N + j2) * M + m)) : C(0);
+      C u[N];
+#pragma unroll
+      for (int i2 = 0; i2 < N; i2++)
+      {
+        C t = 0;
+#pragma unroll
+        for (int j2 = 0; j2 < N; j2++)
+          t = madd<EXACT>(t, static_cast<C>(cld(A2 + (i2 + j2 * a.lda))), w[j2]);
+        u[i2] = static_cast<C>(static_cast<T>(t));
+      }
 #pragma unroll
       for (int i1 = 0; i1 < N; i1++)
       {
-        if (a.owned)
-        {
-          T yv[N];
+        const C a1 = static_cast<C>(cld(A1 + (i1 + j1 * a.lda)));
 #pragma unroll
-          for (int i2 = 0; i2 < N; i2++)
-            yv[i2] = gld(yb + (long long)(i1 * N + i2) * M + m);
-#pragma unroll
-          for (int i2 = 0; i2 < N; i2++)
-            gst(yb + (long long)(i1 * N + i2) * M + m,
-                yv[i2] + static_cast<T>(z[i1][i2]));
-        }
-        else
-        {
-#pragma unroll
-          for (int i2 = 0; i2 < N; i2++)
-            gatomic_add(yb + (long long)(i1 * N + i2) * M + m,
-                        static_cast<T>(z[i1][i2]));
-        }
+        for (int i2 = 0; i2 < N; i2++)
+          acc[i1][i2] = madd<EXACT>(acc[i1][i2], a1, u[i2]);
       }
     }
   }
+  if (target)
+    flush(target);
 }
 
 // ---------------------------------------------------------------------------
