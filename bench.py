@@ -12,11 +12,13 @@ entry, test_kronmult6_batched.cpp:270), the HBM roofline of the kernel
 reference's CPU path timed on this machine's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode atomic|plan]
-                    [--workload d6n6|d3n8|d6n4|d6n8]
+                    [--workload d6n6|d3n8|d6n4|d6n8|xpool]
 
---workload selects another uniform pbatched configuration for profiling
-(default d6n6, the one above): d3n8 = d=3 n=8 x262144 4:1, d6n4 = d=6 n=4
-x65536 16:1, d6n8 = d=6 n=8 x1024 2:1 (workspace kernels).
+--workload selects another configuration for profiling (default d6n6, the one
+above): d3n8 = d=3 n=8 x262144 4:1, d6n4 = d=6 n=4 x65536 16:1, d6n8 = d=6
+n=8 x1024 2:1 (workspace kernels); xpool = the pooled sparse-grid apply
+through kronmult6_xbatched: d=6, n=4, 1,048,576 entries, 512 pooled 4x4 A per
+mode, 65536 X and 65536 Y vectors, X drawn uniformly, 16 entries per output.
 """
 from __future__ import annotations
 
@@ -31,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+FP64_PEAK_TFLOPS = 78.6  # MI355X published dense FP64 (vector and matrix)
 SEED = 20261203
 # name -> (d, n, batch, overlap): pY[k] = Y + (k/overlap)*n^d
 WORKLOADS = {
@@ -39,6 +42,9 @@ WORKLOADS = {
     "d6n4": (6, 4, 65536, 16),
     "d6n8": (6, 8, 1024, 2),
 }
+# pooled xbatched: entries, A matrices per mode, X vectors, Y vectors, entries per Y
+XPOOL = dict(d=6, n=4, batch=1 << 20, pool=512, nx=65536, ny=65536, fan_in=16)
+XPOOL_CPU_SAMPLE = 16384
 
 
 def flops_per_call(d, n, batch):
@@ -106,7 +112,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="d6n6",
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["xpool"], default="d6n6",
                     help="configuration (default d6n6: d=6, n=6, 4096 entries, 4:1)")
     args = ap.parse_args()
 
@@ -122,6 +128,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload == "xpool":
+        bench_xpool(args, dev)
+        return
     d, n, batch, overlap = WORKLOADS[args.workload]
     s = n ** d
     n_y = batch // overlap
@@ -220,6 +229,155 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "wall_s": round(wall, 4),
+    }
+    print(json.dumps(out))
+    if plan is not None:
+        plan.close()
+
+
+def xpool_cpu_baseline(pools, aidx, X, xidx, yidx, reps=3):
+    """The reference's CPU kronmult6_xbatched (oracle/_ref, default recursive
+    build) when it was built, else the oracle restatement, on the first
+    XPOOL_CPU_SAMPLE entries of the pooled workload (their X and Y vectors
+    gathered into compact host arrays); median of `reps` calls.  ``X`` is the
+    device X pool; only the sampled rows are copied to the host."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as O
+    use_ref = "recursive" if O.ref_available("recursive") else None
+    threads = O.num_threads()
+    m = XPOOL_CPU_SAMPLE
+    d, n = XPOOL["d"], XPOOL["n"]
+    xs, xmap = np.unique(xidx[:m], return_inverse=True)
+    ys, ymap = np.unique(yidx[:m], return_inverse=True)
+    Xs = X[torch.as_tensor(xs, device=X.device)].cpu().numpy()
+    Ys = np.zeros((len(ys), X.shape[1]))
+    times = []
+    for _ in range(reps):
+        O.xbatched(pools, aidx[:m], Xs, xmap, Ys, ymap, use_ref=use_ref,
+                   variant=O.RECURSIVE, timings=times)
+    t = statistics.median(times)
+    return {
+        "value": round(flops_per_call(d, n, m) / t / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": threads,
+        "kind": "reference" if use_ref else "port",
+        "sample": f"first {m} of {XPOOL['batch']} entries ({len(xs)} distinct X, {len(ys)} distinct Y), "
+                  f"median of {reps} calls of kronmult{d}_xbatched (default recursive build), "
+                  f"OpenMP threads={threads}",
+    }
+
+
+def bench_xpool(args, dev):
+    """Pooled sparse-grid apply through kronmult6_xbatched (lda = n).
+
+    Entry e uses A_k = pool_k[aidx[e, k]], reads X[xidx[e]] (uniform over the
+    X vectors) and accumulates into Y[yidx[e]], yidx = a seeded permutation of
+    the entries // fan_in, so every output receives fan_in entries.  The
+    compulsory bytes (every X and Y vector once) put this workload above the
+    FP64 ridge (32 flop/B against 78.6 TF / 8 TB/s = 9.8), so its roofline is
+    the FP64 rate; the bytes with X read once per entry are reported too."""
+    import numpy as np
+    import torch
+
+    import kronmult_amd as km
+
+    c = XPOOL
+    d, n, batch = c["d"], c["n"], c["batch"]
+    s = n ** d
+    g = torch.Generator(device=dev)
+    g.manual_seed(SEED)
+    pools = [torch.rand((c["pool"], n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1
+             for _ in range(d)]
+    X = torch.rand((c["nx"], s), generator=g, device=dev, dtype=torch.float64) * 2 - 1
+    Y = torch.zeros((c["ny"], s), device=dev, dtype=torch.float64)
+    rng = np.random.default_rng(SEED)
+    aidx = rng.integers(0, c["pool"], (batch, d))
+    xidx = rng.integers(0, c["nx"], batch)
+    yidx = rng.permutation(batch) // c["fan_in"]
+    table = km.matrix_pointer_table(pools, aidx)
+    pX = km.pointer_array(X, xidx)
+    pY = km.pointer_array(Y, yidx)
+    plan = km.Plan(d, n, 8, pY, batch) if args.mode in ("best", "plan") else None
+    modes = {"atomic": None, "plan": km.options(plan=plan) if plan else None}
+    todo = ["atomic", "plan"] if args.mode == "best" else [args.mode]
+    fam, nt = km.query_kernel(d, n, 8, None)
+    kernel_name = {0: "kron_lds", 4: "kron_reg"}.get(fam, str(fam))
+    stream = torch.cuda.current_stream()
+
+    def timed(opt):
+        def step():
+            km.kronmult_xbatched(d, n, table, n, pX, pY, None, batch, stream=stream, opt=opt)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / args.steps, time.perf_counter() - t0
+
+    results = {m: timed(modes[m]) for m in todo}
+    best = min(results, key=lambda m: results[m][0])
+    ms_per_step, wall = results[best]
+    fl = flops_per_call(d, n, batch)
+    tflops = fl / (ms_per_step * 1e-3) / 1e12
+    by_compulsory = 8.0 * (c["nx"] * s + c["pool"] * d * n * n + 2 * c["ny"] * s)
+    by_streamed = 8.0 * (batch * s + c["pool"] * d * n * n + 2 * c["ny"] * s)
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = xpool_cpu_baseline([p.cpu().numpy() for p in pools], aidx, X, xidx, yidx)
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"{kernel_name}_xpool_d{d}n{n}_b{batch}.txt"))
+    out = {
+        "metric": "d=6 batched kronmult GFLOP/s + % HBM roofline; mixed vs FP64 at rel err <=1e-10",
+        "value": round(fl / (ms_per_step * 1e-3) / 1e9, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U[-1,1) A pools and X on device, seeded indices)",
+        "config": {
+            "workload": f"kronmult{d}_xbatched FP64 pooled, d={d}, n={n}, batch={batch}, {c['pool']} A per "
+                        f"mode, {c['nx']} X (uniform), {c['ny']} Y ({c['fan_in']} entries each)",
+            "d": d, "n": n, "batch": batch,
+            "reduction": {"plan": "plan (deterministic output-sorted, no atomics)",
+                          "atomic": "atomic adds (drop-in)"}[best],
+            "ms_per_step_by_reduction": {m: round(v[0], 5) for m, v in results.items()},
+            "kernel": f"{kernel_name} ({nt} threads/workgroup)",
+            "hbm_gbs_compulsory": round(by_compulsory / (ms_per_step * 1e-3) / 1e9, 1),
+            "hbm_gbs_x_per_entry": round(by_streamed / (ms_per_step * 1e-3) / 1e9, 1),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(tflops, 2),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
             "traffic": traffic,
         },
         "cpu_baseline": cpu,

@@ -213,14 +213,15 @@ def pbatched(A: np.ndarray, X: np.ndarray, Y: np.ndarray, yidx: Sequence[int],
 
 def xbatched(Apool: Sequence[np.ndarray], aidx: np.ndarray, X: np.ndarray, xidx: Sequence[int],
              Y: np.ndarray, yidx: Sequence[int], subbatch: int = 0, variant: int = RECURSIVE,
-             use_ref: Optional[str] = None) -> np.ndarray:
+             use_ref: Optional[str] = None, timings: Optional[list] = None) -> np.ndarray:
     """kronmultN_xbatched with pooled A.
 
     ``Apool[k]`` has shape ``(P_k, n, lda)`` indexed ``[p, j, i]`` (column
     major, leading dimension lda >= n); ``aidx`` is ``(batch, d)`` and entry b
     uses ``Apool[k][aidx[b, k]]`` as A_k (the reference's (d,batch) pointer
     table, kronmult_xbatched.hpp:67-69).  X is preserved (copied into the
-    workspace pairs, kronmult_xbatched.hpp:110-119)."""
+    workspace pairs, kronmult_xbatched.hpp:110-119).  When ``timings`` is a
+    list, the wall time of the library call alone is appended to it."""
     aidx = np.asarray(aidx)
     batch, d = aidx.shape
     n = Apool[0].shape[1]
@@ -244,11 +245,14 @@ def xbatched(Apool: Sequence[np.ndarray], aidx: np.ndarray, X: np.ndarray, xidx:
     pY = _rowptrs(Yo, yidx)
     pW = _rowptrs(W, np.arange(nW))
     suf = _suffix(dt)
+    t0 = time.perf_counter()
     if use_ref:
         rc = getattr(ref_lib(use_ref), f"ref_xbatched_{suf}")(d, n, _ptr(Aptr), lda, _ptr(pX), _ptr(pY), _ptr(pW), batch, subbatch)
         assert rc == 0
     else:
         getattr(lib(), f"oracle_xbatched_{suf}")(d, n, _ptr(Aptr), lda, _ptr(pX), _ptr(pY), _ptr(pW), batch, subbatch, variant)
+    if timings is not None:
+        timings.append(time.perf_counter() - t0)
     return Yo
 
 
