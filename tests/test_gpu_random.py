@@ -1,0 +1,102 @@
+"""Seeded randomized parity sweep: pbatched and xbatched calls with random
+shapes (every d <= 6, n <= 8 up to n^d = 262144), random X and Y index maps
+(repeats and collisions anywhere), FMA or exact mode, every variant, with or
+without a plan, FP64 or float, and for xbatched a padded lda, a small A pool
+and a random subbatchCount; every call against the CPU oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _helpers import RTOL_FP32, RTOL_FP64, rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(d, n) for d in range(1, 7) for n in range(1, 9) if n ** d <= 262144]
+
+
+def _case(km, rng):
+    import torch
+    dev = torch.device("cuda:0")
+    d, n = SHAPES[rng.integers(len(SHAPES))]
+    s = n ** d
+    bmax = 64 if s <= 5000 else (8 if s <= 50000 else 3)
+    batch = int(rng.integers(1, bmax + 1))
+    f32 = bool(rng.random() < 0.15)
+    dt = np.float32 if f32 else np.float64
+    exact = bool(rng.random() < 0.3)
+    variant = int(rng.integers(0, 3))
+    use_plan = bool(rng.random() < 0.4)
+    xb = bool(rng.random() < 0.3)
+    nY = int(rng.integers(1, batch + 1))
+    yidx = rng.integers(0, nY, batch)
+    nX = int(rng.integers(1, batch + 1))
+    xidx = rng.integers(0, nX, batch)
+    X = rng.uniform(-1, 1, (nX, s)).astype(dt)
+    Y0 = rng.uniform(-1, 1, (nY, s)).astype(dt)
+    desc = dict(xbatched=xb, d=d, n=n, batch=batch, f32=f32, exact=exact, variant=variant, plan=use_plan)
+    tX = torch.tensor(X, device=dev)
+    tY = torch.tensor(Y0, device=dev)
+    pX = km.pointer_array(tX, xidx)
+    pY = km.pointer_array(tY, yidx)
+    plan = km.Plan(d, n, 4 if f32 else 8, pY, batch) if use_plan else None
+    opt = km.options(plan=plan, exact=exact, variant=variant)
+    dts = "f32" if f32 else "f64"
+    ov = O.FORWARD if variant == km.VARIANT_FORWARD else O.BACKWARD
+    # every device buffer a call reads stays referenced until the
+    # synchronize below: a freed tensor's memory goes to the next allocation
+    # while the kernel may still read it
+    try:
+        if not xb:
+            A = rng.uniform(-1, 1, (batch, d, n, n)).astype(dt)
+            tA = torch.tensor(A, device=dev)
+            tW = torch.zeros((batch, s), dtype=tX.dtype, device=dev)
+            pW = km.pointer_array(tW, np.arange(batch))
+            km.kronmult_pbatched(d, n, tA, pX, pY, pW, batch, dtype=dts, opt=opt)
+            want = O.pbatched(A, X, Y0, yidx, xidx=xidx, variant=ov)
+        else:
+            lda = n + int(rng.integers(0, 4))
+            P = int(rng.integers(1, 6))
+            pools = [rng.uniform(-1, 1, (P, n, lda)).astype(dt) for _ in range(d)]
+            aidx = rng.integers(0, P, (batch, d))
+            sub = int(rng.integers(0, batch + 1))
+            tpools = [torch.tensor(p, device=dev) for p in pools]
+            table = km.matrix_pointer_table(tpools, aidx)
+            nW = max(2 * (sub if sub else max(1, batch // 2)), batch)
+            tW = torch.zeros((nW, s), dtype=tX.dtype, device=dev)
+            pW = km.pointer_array(tW, np.arange(nW))
+            try:
+                km.kronmult_xbatched(d, n, table, lda, pX, pY, pW, batch, sub, dtype=dts, opt=opt)
+            except km.KronmultError as e:
+                # a plan cannot cover a workspace that is reused chunk by chunk
+                assert use_plan and e.status == km.ERROR_NOT_SUPPORTED and sub < batch, desc
+                return None
+            want = O.xbatched(pools, aidx, X, xidx, Y0, yidx, subbatch=sub, variant=ov)
+        torch.cuda.synchronize()
+        assert np.array_equal(tX.cpu().numpy(), X), ("X was modified", desc)
+        return desc, rel_err(tY.cpu().numpy(), want), (RTOL_FP32 if f32 else RTOL_FP64)
+    finally:
+        if plan is not None:
+            plan.close()
+
+
+@pytest.fixture(scope="module")
+def km():
+    import kronmult_amd
+    kronmult_amd.lib()
+    return kronmult_amd
+
+
+@pytest.mark.parametrize("seed", [11, 23, 37])
+def test_random_calls_vs_oracle(km, seed):
+    rng = np.random.default_rng(seed)
+    ran = 0
+    for _ in range(200):
+        r = _case(km, rng)
+        if r is None:
+            continue
+        desc, err, tol = r
+        assert err <= tol, (desc, err)
+        ran += 1
+    assert ran >= 190
