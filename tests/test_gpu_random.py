@@ -100,3 +100,33 @@ def test_random_calls_vs_oracle(km, seed):
         assert err <= tol, (desc, err)
         ran += 1
     assert ran >= 190
+
+
+@pytest.mark.parametrize("seed", [5, 17])
+def test_random_vbatched_vs_oracle(km, seed):
+    """Uniform rectangular shapes (m_k, n_k in 1..6, d <= 6), random output
+    maps, FMA or exact, every variant, FP64 or float."""
+    from _helpers import gpu_vbatched
+    rng = np.random.default_rng(seed)
+    for _ in range(60):
+        d = int(rng.integers(1, 7))
+        while True:
+            m = tuple(int(v) for v in rng.integers(1, 7, d))
+            n = tuple(int(v) for v in rng.integers(1, 7, d))
+            if np.prod(m) <= 5000 and np.prod(n) <= 5000:
+                break
+        batch = int(rng.integers(1, 17))
+        nY = int(rng.integers(1, batch + 1))
+        yidx = rng.integers(0, nY, batch)
+        f32 = bool(rng.random() < 0.2)
+        dt = np.float32 if f32 else np.float64
+        exact = bool(rng.random() < 0.3)
+        variant = int(rng.integers(0, 3))
+        mats = [[rng.uniform(-1, 1, (n[k], m[k])).astype(dt) for k in range(d)] for _ in range(batch)]
+        X = rng.uniform(-1, 1, (batch, int(np.prod(n)))).astype(dt)
+        Y0 = rng.uniform(-1, 1, (nY, int(np.prod(m)))).astype(dt)
+        got = gpu_vbatched(m, n, mats, X, Y0, yidx, opt=km.options(exact=exact, variant=variant))
+        ov = {km.VARIANT_BACKWARD: O.BACKWARD, km.VARIANT_FORWARD: O.RECURSIVE, km.VARIANT_ALG993: O.ALG993}[variant]
+        want = O.vbatched(m, n, mats, X, Y0, yidx, variant=ov)
+        tol = RTOL_FP32 if f32 else RTOL_FP64
+        assert rel_err(got, want) <= tol, (m, n, batch, f32, exact, variant, rel_err(got, want))
