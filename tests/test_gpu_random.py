@@ -130,3 +130,24 @@ def test_random_vbatched_vs_oracle(km, seed):
         want = O.vbatched(m, n, mats, X, Y0, yidx, variant=ov)
         tol = RTOL_FP32 if f32 else RTOL_FP64
         assert rel_err(got, want) <= tol, (m, n, batch, f32, exact, variant, rel_err(got, want))
+
+
+@pytest.mark.parametrize("seed", [3, 29])
+def test_random_grouped_vs_oracle(km, seed):
+    """Grouped calls with random d, random per-entry sizes (within each d's
+    supported range) and random output groups, in every launch mode."""
+    import test_gpu_grouped as G
+    rng = np.random.default_rng(seed)
+    nmax = {1: 8, 2: 8, 3: 8, 4: 8, 5: 6, 6: 4}
+    for _ in range(15):
+        d = int(rng.integers(1, 7))
+        k = int(rng.integers(1, 5))
+        sizes = sorted({int(v) for v in rng.integers(1, nmax[d] + 1, k)})
+        batch = int(rng.integers(1, 120))
+        ngroups = int(rng.integers(1, 8))
+        mode = list(G.MODES)[int(rng.integers(len(G.MODES)))]
+        ns, yg, mats, X, Y0 = G._make(int(rng.integers(1 << 30)), d, sizes, batch, ngroups)
+        want = G._oracle(d, ns, yg, mats, X, Y0)
+        outs, _ = G._gpu(km, d, ns, yg, mats, X, Y0, mode=mode)
+        for nn in want:
+            assert rel_err(outs[0][nn], want[nn]) <= RTOL_FP64, (d, sizes, batch, ngroups, mode, nn)
