@@ -103,3 +103,14 @@ def test_query_kernel_families():
     assert km.query_kernel(6, 8)[0] == km.FAMILY_TWOPASS
     assert km.query_kernel(3, 12)[0] == km.FAMILY_GLOBAL
     assert km.query_kernel(6, 4, opt=km.options(force_global=True))[0] == km.FAMILY_GLOBAL
+
+
+def test_cpp_caller_of_the_reference_interface_links(tmp_path):
+    """A C++ program that includes only kronmult_compat.hpp and the HIP
+    runtime (tests/compat/caller.cpp) compiles and links against the library
+    with the reference's function names and argument lists."""
+    out = tmp_path / "caller"
+    r = subprocess.run(["make", "-B", "-C", os.path.join(ROOT, "tests", "compat"), f"OUT={out}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert out.exists()
