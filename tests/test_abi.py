@@ -114,3 +114,35 @@ def test_cpp_caller_of_the_reference_interface_links(tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
     assert out.exists()
+
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include "kronmult_amd.h"
+int main(void)
+{
+  km_options o;
+  int m[6] = {6, 6, 6, 6, 6, 6};
+  km_options_init(&o);
+  printf("%d %d %s %.0f\n", (int)o.precision, (int)o.variant,
+         km_status_string(KM_ERROR_INVALID_VALUE), km_kron_flops(6, m, m));
+  return 0;
+}
+"""
+
+
+def test_c_abi_header_is_plain_c(tmp_path):
+    """kronmult_amd.h compiles as pedantic C99 (what cgo, JNI and other FFI
+    hosts see), links with gcc and its host-only functions run without a GPU."""
+    src = tmp_path / "c_abi.c"
+    src.write_text(C_PROGRAM)
+    exe = tmp_path / "c_abi"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L", os.path.dirname(LIB), "-lkronmult_amd", "-L/opt/rocm/lib", "-lamdhip64",
+                        f"-Wl,-rpath,{os.path.dirname(LIB)}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    # defaults FP64 and backward order, the status text, the flop model
+    assert out.stdout.strip() == f"0 0 invalid value {12 * 6 ** 7}"
