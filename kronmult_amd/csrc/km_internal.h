@@ -28,6 +28,9 @@ struct KernelInfo
 
 // threads per workgroup of kron_lds for a tensor of s elements
 constexpr int lds_threads(int s) { return s <= 1024 ? 64 : (s <= 4096 ? 256 : 512); }
+// wider workgroups for size bins that fit a few resident waves of
+// workgroups, where one entry's latency bounds the launch (grouped calls)
+constexpr int lds_threads_wide(int s) { return s <= 256 ? 64 : (s <= 2048 ? 256 : 512); }
 // largest tensor kron_lds keeps in LDS (doubles): 150 KiB of the 160 KiB
 constexpr int lds_max_elems(int elem_bytes) { return 150 * 1024 / elem_bytes; }
 
@@ -37,6 +40,10 @@ bool lds_kernel(int d, int n, bool exact, bool fwd, KernelInfo *out);
 // per-d units
 template<typename T, int D>
 bool lds_kernel_d(int n, bool exact, bool fwd, KernelInfo *out);
+// FMA kron_lds with lds_threads_wide threads, for the (d, n) where that
+// differs from lds_threads and no kron_reg exists (km_lds_wide.hip)
+template<typename T>
+bool lds_kernel_wide(int d, int n, KernelInfo *out);
 
 // batched slender GEMM (km_kgemm.hip); op 0 = nn, 1 = nt, 2 = tt
 template<typename T>
