@@ -250,6 +250,16 @@ bool bin_kernel(int d, int n, bool exact, KernelInfo *k)
          (k->family == km::FAM_LDS || k->family == km::FAM_REG);
 }
 
+// A grouped size bin whose entries fit within this many resident waves of
+// its kernel's workgroups is bound by one entry's chain of loads and passes:
+// FMA calls served by kron_lds then use the wider kron_lds
+// (lds_kernel_wide).  Measured on the grouped d=4, 20000-entry, n = 2..8
+// call: 0.1505 -> 0.1413 ms with 4 (0.146 with 2, the same with 8).  Plain
+// launches keep the narrow kernels: there the rule won on small batches
+// (d5n5 x1000 0.026 -> 0.020 ms) but lost on others (d3n7 x5000 0.016 ->
+// 0.021 ms, d4n5 x8000 0.028 -> 0.035 ms).
+constexpr long long kWideWaves = 4;
+
 // Whether a call's X vectors, each read once, are loaded with the
 // non-temporal hint (EntryArgs::x_stream).  A call whose X fits well inside
 // the 256 MB memory-side cache keeps the default policy: a repeated call
@@ -1297,12 +1307,8 @@ extern "C"
                                           : bin_kernel<float>(d, n, false, &kb);
       const long long cnt = j - i;
       long long g = have ? (long long)device_cus() * occupancy(kb.fn, kb.nt) : 1;
-      // a bin whose entries fit a few resident waves of workgroups is bound
-      // by one entry's latency: FMA calls then take the wider kron_lds, and
-      // the split is sized for it.  Measured on d=4, 20000 entries,
-      // n = 2..8 (bins n = 5 and 7 widen): 0.1505 -> 0.1413 ms with up to
-      // 4 waves (0.146 with up to 2, the same with up to 8).
-      constexpr long long kWideWaves = 4;
+      // a bin that fits kWideWaves resident waves takes the wider kron_lds
+      // for FMA calls, and its split is sized for it
       km_group_bin bn;
       bn.wide = 0;
       KernelInfo kw;
