@@ -12,11 +12,16 @@ entry, test_kronmult6_batched.cpp:270), the HBM roofline of the kernel
 reference's CPU path timed on this machine's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode atomic|plan]
-                    [--workload d6n6|d3n8|d6n4|d6n8|xpool]
+                    [--workload d6n6|d3n8|d6n4|d6n8|d6n4-interleaved|grouped|xpool]
 
 --workload selects another configuration for profiling (default d6n6, the one
 above): d3n8 = d=3 n=8 x262144 4:1, d6n4 = d=6 n=4 x65536 16:1, d6n8 = d=6
-n=8 x1024 2:1 (workspace kernels); xpool = the pooled sparse-grid apply
+n=8 x1024 2:1 (workspace kernels); d6n4-interleaved = d=6 n=4 x512 with
+pY[k] = Y + (k mod 64)*n^d (8 entries per output, spread over the batch);
+grouped = d=4, 20000 entries with n drawn from 2..8, 285 output groups per n,
+each entry adding into a random group of its n, through the grouped entry
+point (compared with the deterministic and persistent modes and with one
+kronmult4_vbatched call per n); xpool = the pooled sparse-grid apply
 through kronmult6_xbatched: d=6, n=4, 1,048,576 entries, 512 pooled 4x4 A per
 mode, 65536 X and 65536 Y vectors, X drawn uniformly, 16 entries per output.
 """
@@ -41,7 +46,13 @@ WORKLOADS = {
     "d3n8": (3, 8, 262144, 4),
     "d6n4": (6, 4, 65536, 16),
     "d6n8": (6, 8, 1024, 2),
+    "d6n4-interleaved": (6, 4, 512, 8),
 }
+# workloads whose outputs interleave: pY[k] = Y + (k mod (batch/overlap))*n^d
+INTERLEAVED = {"d6n4-interleaved"}
+# grouped: d, entries, sizes, output groups per size
+GROUPED = dict(d=4, batch=20000, sizes=(2, 3, 4, 5, 6, 7, 8), groups=285)
+GROUPED_CPU_SAMPLE = 20000
 # pooled xbatched: entries, A matrices per mode, X vectors, Y vectors, entries per Y
 XPOOL = dict(d=6, n=4, batch=1 << 20, pool=512, nx=65536, ny=65536, fan_in=16)
 XPOOL_CPU_SAMPLE = 16384
@@ -112,7 +123,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["xpool"], default="d6n6",
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["grouped", "xpool"], default="d6n6",
                     help="configuration (default d6n6: d=6, n=6, 4096 entries, 4:1)")
     args = ap.parse_args()
 
@@ -131,6 +142,9 @@ def main():
     if args.workload == "xpool":
         bench_xpool(args, dev)
         return
+    if args.workload == "grouped":
+        bench_grouped(args, dev)
+        return
     d, n, batch, overlap = WORKLOADS[args.workload]
     s = n ** d
     n_y = batch // overlap
@@ -140,7 +154,7 @@ def main():
     A = torch.rand((batch, d, n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1
     X = torch.rand((batch, s), generator=g, device=dev, dtype=torch.float64) * 2 - 1
     Y = torch.zeros((n_y, s), device=dev, dtype=torch.float64)
-    yidx = np.arange(batch) // overlap
+    yidx = (np.arange(batch) % n_y) if args.workload in INTERLEAVED else (np.arange(batch) // overlap)
     pX = km.pointer_array(X, np.arange(batch))
     pY = km.pointer_array(Y, yidx)
     fam, nt = km.query_kernel(d, n, 8, None)
@@ -215,7 +229,8 @@ def main():
         "data": "synthetic (seeded U[-1,1) A and X on device)",
         "config": {
             "workload": f"kronmult{d}_pbatched FP64, d={d}, n={n}, batch={batch}, {overlap}:1 output "
-                        f"overlap (pY[k] = Y + (k/{overlap})*n^d)",
+                        + (f"overlap (pY[k] = Y + (k mod {n_y})*n^d)" if args.workload in INTERLEAVED
+                           else f"overlap (pY[k] = Y + (k/{overlap})*n^d)"),
             "d": d, "n": n, "batch": batch, "overlap": overlap,
             "reduction": {"plan": "plan (deterministic output-sorted, no atomics)",
                           "atomic": "atomic adds (drop-in)"}[best],
@@ -386,6 +401,158 @@ def bench_xpool(args, dev):
     print(json.dumps(out))
     if plan is not None:
         plan.close()
+
+
+def bench_grouped(args, dev):
+    """Grouped variable-size batch through kronmult_grouped (one call per
+    step: one on-chip launch per size bin).  Entry e has its own n_e drawn
+    uniformly from GROUPED['sizes'], its own four n_e x n_e A ~ U[-1,1) and
+    X ~ U[-1,1); the outputs are GROUPED['groups'] vectors per size and each
+    entry adds into a random one of its size.  The same call with
+    KM_FLAG_DETERMINISTIC and with KM_FLAG_PERSISTENT, and one
+    kronmult4_vbatched call per size, are timed beside it."""
+    import numpy as np
+    import torch
+
+    import kronmult_amd as km
+
+    c = GROUPED
+    d, batch, sizes, groups = c["d"], c["batch"], c["sizes"], c["groups"]
+    rng = np.random.default_rng(SEED)
+    ns = rng.choice(sizes, batch)
+    ys = rng.integers(0, groups, batch)
+    g = torch.Generator(device=dev)
+    g.manual_seed(SEED)
+    idx = {n: np.flatnonzero(ns == n) for n in sizes}
+    X = {n: torch.rand((len(idx[n]), n ** d), generator=g, device=dev, dtype=torch.float64) * 2 - 1 for n in sizes}
+    A = {n: torch.rand((len(idx[n]), d, n, n), generator=g, device=dev, dtype=torch.float64) * 2 - 1 for n in sizes}
+    Y = {n: torch.zeros((groups, n ** d), device=dev, dtype=torch.float64) for n in sizes}
+    # entry e is the slot-th entry of its size
+    slot = np.empty(batch, dtype=np.int64)
+    for n in sizes:
+        slot[idx[n]] = np.arange(len(idx[n]))
+    xaddr = np.array([X[n].data_ptr() + slot[e] * X[n].stride(0) * 8 for e, n in enumerate(ns)], dtype=np.int64)
+    yaddr = np.array([Y[n].data_ptr() + ys[e] * Y[n].stride(0) * 8 for e, n in enumerate(ns)], dtype=np.int64)
+    aaddr = np.array([[A[n].data_ptr() + (slot[e] * A[n].stride(0) + k * A[n].stride(1)) * 8 for k in range(d)]
+                      for e, n in enumerate(ns)], dtype=np.int64).reshape(-1)
+    pX = torch.tensor(xaddr, device=dev)
+    pY = torch.tensor(yaddr, device=dev)
+    table = torch.tensor(aaddr, device=dev)
+    plan = km.GroupPlan(d, [int(v) for v in ns], pY)
+    # the same entries as one kronmult4_vbatched call per size
+    per_n = {n: (torch.tensor(xaddr[idx[n]], device=dev), torch.tensor(yaddr[idx[n]], device=dev),
+                 torch.tensor(aaddr.reshape(batch, d)[idx[n]].reshape(-1), device=dev)) for n in sizes}
+    stream = torch.cuda.current_stream()
+    calls = {
+        "grouped": lambda: km.kronmult_grouped(plan, table, pX, pY, stream=stream),
+        "grouped_deterministic": lambda: km.kronmult_grouped(plan, table, pX, pY, stream=stream,
+                                                             opt=km.options(deterministic=True)),
+        "grouped_persistent": lambda: km.kronmult_grouped(plan, table, pX, pY, stream=stream,
+                                                          opt=km.options(persistent=True)),
+        "vbatched_per_size": lambda: [km.kronmult_vbatched([n] * d, [n] * d, per_n[n][2], per_n[n][0], per_n[n][1],
+                                                           None, 0, len(idx[n]), stream=stream) for n in sizes],
+    }
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / args.steps, time.perf_counter() - t0
+
+    # --mode atomic / plan time the default or the deterministic call alone
+    # (for profiling); the default mode times all four
+    only = {"atomic": "grouped", "plan": "grouped_deterministic"}.get(args.mode)
+    results = {name: timed(fn) for name, fn in calls.items() if only in (None, name)}
+    ms_per_step, wall = results[only or "grouped"]
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+    fl = float(sum(2.0 * d * int(n) ** (d + 1) for n in ns))
+    by = 8.0 * (sum(int(n) ** d + d * int(n) ** 2 for n in ns) + 2 * groups * sum(n ** d for n in sizes))
+    gbs = by / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        # the reference's CPU kronmult4_pbatched, one call per size, on the
+        # first GROUPED_CPU_SAMPLE entries
+        from oracle import oracle as O
+        use_ref = "recursive" if O.ref_available("recursive") else None
+        m = GROUPED_CPU_SAMPLE
+        times = []
+        for n in sizes:
+            sel = idx[n][idx[n] < m]
+            if len(sel) == 0:
+                continue
+            s_ = slot[sel]
+            An = A[n][torch.as_tensor(s_, device=dev)].cpu().numpy()
+            Xn = X[n][torch.as_tensor(s_, device=dev)].cpu().numpy()
+            Yn = np.zeros((groups, n ** d))
+            t = []
+            for _ in range(3):
+                if use_ref:
+                    O.pbatched(An, Xn, Yn, ys[sel], use_ref=use_ref, timings=t)
+                else:
+                    O.pbatched(An, Xn, Yn, ys[sel], variant=O.RECURSIVE, timings=t)
+            times.append(statistics.median(t))
+        fl_s = float(sum(2.0 * d * int(n) ** (d + 1) for n in ns[:m]))
+        cpu = {
+            "value": round(fl_s / sum(times) / 1e9, 3),
+            "unit": "GFLOP/s",
+            "cores": O.num_threads(),
+            "kind": "reference" if use_ref else "port",
+            "sample": f"first {m} of {batch} entries, one kronmult{d}_pbatched call per size "
+                      f"(default recursive build), median of 3 per size, OpenMP threads={O.num_threads()}",
+        }
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"grouped_d{d}_b{batch}.txt"))
+    out = {
+        "metric": "d=6 batched kronmult GFLOP/s + % HBM roofline; mixed vs FP64 at rel err <=1e-10",
+        "value": round(fl / (ms_per_step * 1e-3) / 1e9, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded sizes, groups, U[-1,1) A and X on device)",
+        "config": {
+            "workload": f"kronmult_grouped FP64, d={d}, {batch} entries, n in {list(sizes)}, {groups} output "
+                        f"groups per n, each entry into a random group of its n",
+            "d": d, "batch": batch,
+            "reduction": ("owned outputs (KM_FLAG_DETERMINISTIC)" if only == "grouped_deterministic"
+                          else "atomic adds (per-size launches, split evenly)"),
+            "ms_per_step_by_call": {k: round(v[0], 5) for k, v in results.items()},
+            "kernel": "one kron_reg / kron_lds launch per size bin",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "wall_s": round(wall, 4),
+    }
+    print(json.dumps(out))
+    plan.close()
 
 
 if __name__ == "__main__":

@@ -87,3 +87,20 @@ def test_pooled_cpu_leg_on_a_small_sample(monkeypatch):
     assert r["value"] > 0 and r["unit"] == "GFLOP/s" and r["cores"] >= 1
     assert r["kind"] == ("reference" if O.ref_available("recursive") else "port")
     assert r["sample"].startswith("first 64 of ")
+
+
+def test_profiled_traffic_of_the_grouped_workload():
+    c = bench.GROUPED
+    rng = np.random.default_rng(bench.SEED)
+    ns = rng.choice(c["sizes"], c["batch"])
+    d = c["d"]
+    alg = 8.0 * (sum(int(n) ** d + d * int(n) ** 2 for n in ns) + 2 * c["groups"] * sum(n ** d for n in c["sizes"]))
+    t = bench.parse_traffic(os.path.join(ROOT, "profiles", f"grouped_d{d}_b{c['batch']}.txt"))
+    assert t is not None
+    # reads match the algorithmic bytes; shared runs add Y writes
+    assert 1.0 <= t / alg <= 1.4
+
+
+def test_workload_names():
+    assert set(bench.WORKLOADS) >= {"d6n6", "d3n8", "d6n4", "d6n8", "d6n4-interleaved"}
+    assert bench.INTERLEAVED <= set(bench.WORKLOADS)
