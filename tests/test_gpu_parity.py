@@ -321,7 +321,7 @@ def test_plan_deterministic_and_correct(km, d, n, pattern):
     if pattern == "mod":
         yidx = np.arange(batch) % nY           # interleaved outputs: k mod nY
     elif pattern == "block":
-        yidx = np.arange(batch) * nY // batch  # C2-style: consecutive blocks
+        yidx = np.arange(batch) * nY // batch  # consecutive blocks of equal outputs
     else:
         yidx = np.random.default_rng(3).integers(0, nY, batch)
     dev = torch.device("cuda:0")
