@@ -151,3 +151,34 @@ def test_random_grouped_vs_oracle(km, seed):
         outs, _ = G._gpu(km, d, ns, yg, mats, X, Y0, mode=mode)
         for nn in want:
             assert rel_err(outs[0][nn], want[nn]) <= RTOL_FP64, (d, sizes, batch, ngroups, mode, nn)
+
+
+@pytest.mark.parametrize("d,sizes", [(4, [2, 3, 4, 5, 6, 7, 8]), (3, [1, 7, 8]), (5, [3, 4, 5]), (6, [2, 3, 4])])
+@pytest.mark.parametrize("mode", ["atomic", "deterministic", "persistent"])
+def test_grouped_float(km, d, sizes, mode):
+    """The float grouped entry point (float data, double accumulation) against
+    the FP64 oracle on the same float inputs, in every launch mode."""
+    import torch
+    import test_gpu_grouped as G
+    dev = torch.device("cuda:0")
+    ns, yg, mats, X, Y0 = G._make(7 * d + len(sizes), d, sizes, 240, 4)
+    mats = [[m.astype(np.float32) for m in ms] for ms in mats]
+    X = [x.astype(np.float32) for x in X]
+    Y0 = {n: v.astype(np.float32) for n, v in Y0.items()}
+    want = G._oracle(d, ns, yg, [[m.astype(np.float64) for m in ms] for ms in mats],
+                     [x.astype(np.float64) for x in X], {n: v.astype(np.float64) for n, v in Y0.items()})
+    batch = len(ns)
+    tY = {n: torch.tensor(v, device=dev) for n, v in Y0.items()}
+    tX = [torch.tensor(x, device=dev) for x in X]
+    tA = [torch.tensor(np.ascontiguousarray(mats[b][k].T), device=dev) for b in range(batch) for k in range(d)]
+    table = torch.tensor([t.data_ptr() for t in tA], dtype=torch.int64, device=dev)
+    pX = torch.tensor([t.data_ptr() for t in tX], dtype=torch.int64, device=dev)
+    pY = torch.tensor([tY[n][yg[b]].data_ptr() for b, n in enumerate(ns)], dtype=torch.int64, device=dev)
+    plan = km.GroupPlan(d, list(ns), pY, elem_bytes=4)
+    try:
+        km.kronmult_grouped(plan, table, pX, pY, dtype="f32", opt=km.options(**G.MODES[mode]))
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+    for n in want:
+        assert rel_err(tY[n].cpu().numpy(), want[n]) <= RTOL_FP32, (d, n, mode)
