@@ -622,3 +622,28 @@ def test_graph_capture_replays(km, d, n):
         assert torch.allclose(tY - tY0, eager, rtol=0, atol=1e-12 * float(eager.abs().max())), rep
     want = O.pbatched(A, X, Y0, np.arange(batch) // 4, variant=O.BACKWARD)
     assert rel_err(tY.cpu().numpy(), want) <= RTOL_FP64
+
+
+@pytest.mark.parametrize("d,n", [(4, 5), (6, 6), (6, 8)])
+def test_explicit_stream(km, d, n):
+    """A call on a caller-created stream (default priority) runs there: the
+    result is complete once that stream alone is synchronized, and equals the
+    oracle's."""
+    import torch
+    dev = torch.device("cuda:0")
+    batch = 12 if n ** d < 50000 else 4
+    A, X, Y0 = rng_inputs(950 + d * n, d, n, batch, batch // 2)
+    yidx = np.arange(batch) // 2
+    s = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(s):
+        tA = torch.tensor(A, device=dev)
+        tX = torch.tensor(X, device=dev)
+        tY = torch.tensor(Y0, device=dev)
+        tW = torch.zeros_like(tX)
+        pX = km.pointer_array(tX, np.arange(batch))
+        pY = km.pointer_array(tY, yidx)
+        pW = km.pointer_array(tW, np.arange(batch))
+    km.kronmult_pbatched(d, n, tA, pX, pY, pW, batch, stream=s)
+    s.synchronize()
+    want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
+    assert rel_err(tY.cpu().numpy(), want) <= RTOL_FP64
