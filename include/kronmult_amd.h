@@ -74,6 +74,12 @@ typedef enum km_precision
 #define KM_FLAG_DETERMINISTIC 0x8u /* grouped calls: every output owned by one
                                       workgroup, no atomics, bitwise
                                       reproducible run to run                */
+#define KM_FLAG_PLAN_OUTPUTS 0x10u /* calls with an output-ordered plan (or a
+                                      group plan in an owned mode) write to
+                                      the outputs recorded when the plan was
+                                      created and do not read pY; without it
+                                      pY is checked on the device (see
+                                      km_plan_create)                        */
 
 typedef struct km_plan_s *km_plan;
 
@@ -170,8 +176,17 @@ km_status km_kronmult_vbatched_f32(int d, const int *m, const int *n,
  * workgroups, so that every output vector is owned by exactly one workgroup:
  * no atomics, bitwise reproducible results run to run.  No counterpart in
  * the reference (its overlapping outputs use atomics, kgemm_nt.hpp:184-187).
- * Creating a plan reads pY back to the host, sorts, and allocates two small
- * device arrays; a call that passes the plan allocates nothing.
+ * Creating a plan sorts the entries by output on the device (a stable radix
+ * sort; batches above 4,194,304 entries are sorted on the host), keeps a
+ * device copy of pY, allocates a few device arrays and synchronizes
+ * `stream`; a call that passes the plan allocates nothing.
+ * Binding: a plan is built for the pY contents it was created from.  Every
+ * call with an output-ordered plan first compares the call's pY with that
+ * copy on the device (one small kernel on `stream`); if any pointer differs,
+ * the call accumulates into the caller's pY with atomics instead (correct,
+ * not bitwise reproducible).  With KM_FLAG_PLAN_OUTPUTS the check is skipped
+ * and the call writes to the outputs recorded in the plan (pY is not read).
+ * A plan may be used from several streams at once.
  * elem_bytes is sizeof(double) or sizeof(float).
  * ------------------------------------------------------------------------ */
 km_status km_plan_create(km_plan *plan, int d, int n, int elem_bytes,
@@ -217,8 +232,10 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
  * owns, forked from `stream` and joined back into it with events (also
  * capturable), so a plan serves one call at a time; KM_FLAG_PERSISTENT runs all bins in a single persistent
  * launch with owned outputs.  Creating the plan reads the host array n_host
- * and the device pointer array pY once; calls must pass the same pY
- * contents.
+ * and the device pointer array pY once.  The owned modes are bound to those
+ * pY contents as output-ordered plans are (km_plan_create): a call whose pY
+ * differs accumulates with atomics, and KM_FLAG_PLAN_OUTPUTS writes to the
+ * recorded outputs.
  * Supported sizes: n_b <= 8 for d <= 4, n_b <= 6 for d = 5, n_b <= 4 for
  * d = 6 (KM_ERROR_NOT_SUPPORTED otherwise).
  * ------------------------------------------------------------------------ */

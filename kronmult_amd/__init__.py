@@ -46,6 +46,7 @@ FLAG_EXACT = 0x1
 FLAG_FORCE_GLOBAL = 0x2
 FLAG_PERSISTENT = 0x4
 FLAG_DETERMINISTIC = 0x8
+FLAG_PLAN_OUTPUTS = 0x10
 
 # kernel families reported by query_kernel
 FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL, FAMILY_TWOPASS, FAMILY_REG = 0, 1, 2, 3, 4
@@ -180,11 +181,13 @@ def status_string(status: int) -> str:
 
 def options(exact: bool = False, variant: int = VARIANT_BACKWARD, plan: "Optional[Plan]" = None,
             force_global: bool = False, precision: int = PRECISION_FP64,
-            persistent: bool = False, deterministic: bool = False) -> Options:
+            persistent: bool = False, deterministic: bool = False,
+            plan_outputs: bool = False) -> Options:
     o = Options()
     lib().km_options_init(C.byref(o))
     o.flags = ((FLAG_EXACT if exact else 0) | (FLAG_FORCE_GLOBAL if force_global else 0)
-               | (FLAG_PERSISTENT if persistent else 0) | (FLAG_DETERMINISTIC if deterministic else 0))
+               | (FLAG_PERSISTENT if persistent else 0) | (FLAG_DETERMINISTIC if deterministic else 0)
+               | (FLAG_PLAN_OUTPUTS if plan_outputs else 0))
     o.variant = int(variant)
     o.precision = int(precision)
     o.plan = plan.handle if plan is not None else None

@@ -7,6 +7,7 @@
 //   km_kronmult_xbatched_*  kronmult6_xbatched.cpp:4-28
 //   km_kronmult_vbatched_*  kronmult6_vbatched.hpp:10-34 (template only there)
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -32,6 +33,11 @@ struct km_plan_s
   int owned = 1; // 1: output-ordered (owned outputs); 0: input-ordered (atomics)
   int *order = nullptr;  // device, batch entries
   int *ranges = nullptr; // device, ngroups + 1 entries
+  // binding (owned plans): the pY contents the plan was built from, the
+  // check kernel's flag and the generation of the last checked call
+  void **ybind = nullptr; // device, batch pointers
+  int *flag = nullptr;    // device, one int
+  std::atomic<unsigned> gen{0};
 };
 
 struct km_group_bin
@@ -57,6 +63,10 @@ struct km_group_plan_s
   // per-size-bin launch: bin b uses binranges[first .. first+count]
   std::vector<km_group_bin> bins;
   int *binranges = nullptr; // device
+  // binding of the owned launch modes (as km_plan_s)
+  void **ybind = nullptr; // device, batch pointers
+  int *flag = nullptr;    // device, one int
+  std::atomic<unsigned> gen{0};
   // bins run concurrently on these streams (default priority), forked from
   // and joined back into the caller's stream with events
   std::vector<hipStream_t> streams;
@@ -190,6 +200,7 @@ bool alg993_forward(int d, const int *m, const int *n)
 
 struct Resolved
 {
+  unsigned flags = 0;
   bool exact = false;
   bool fwd = false;
   bool force_global = false;
@@ -205,6 +216,7 @@ km_status resolve(const km_options *opt, int d, const int *m, const int *n,
     o = *opt;
   if (o.precision != KM_PRECISION_FP64)
     return KM_ERROR_NOT_SUPPORTED;
+  r->flags = o.flags;
   r->exact = (o.flags & KM_FLAG_EXACT) != 0;
   r->force_global = (o.flags & KM_FLAG_FORCE_GLOBAL) != 0;
   switch (o.variant)
@@ -283,6 +295,34 @@ struct Call
   int elem_bytes = sizeof(T);
 };
 
+// Plan binding, for a call with an output-ordered plan and an output pointer
+// array.  By default a check kernel compares the caller's pY with the
+// pointers the plan was built from; where they differ, the call's kernels
+// give up ownership and accumulate into the caller's pY with atomics
+// (plan_bind).  With KM_FLAG_PLAN_OUTPUTS the call writes to the outputs the
+// plan recorded and does not read pY.  Either way no output is updated by two
+// workgroups without atomics.
+template<typename T>
+km_status bind_plan(Call<T> &c, const Resolved &r, hipStream_t stream)
+{
+  const km_plan p = r.plan;
+  if (!p || !p->owned || !c.a.pY || !p->ybind)
+    return KM_SUCCESS;
+  if (r.flags & KM_FLAG_PLAN_OUTPUTS)
+  {
+    c.a.pY = reinterpret_cast<T *const *>(p->ybind);
+    return KM_SUCCESS;
+  }
+  unsigned g = ++p->gen;
+  if (g == 0) // the flag starts at 0
+    g = ++p->gen;
+  KM_HIP(km::plan_check_launch(reinterpret_cast<const void *const *>(c.a.pY),
+                               p->ybind, c.a.batch, p->flag, g, stream));
+  c.a.bind_flag = p->flag;
+  c.a.bind_gen = g;
+  return KM_SUCCESS;
+}
+
 // Two-pass family: pass 1 over the sub-slabs of a chunk of entries, pass 2
 // over the chunk's entries.  Chunks are as large as the workspace allows:
 // all entries when every entry brings its own workspace (batched,
@@ -310,6 +350,9 @@ km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
     if (chunk_entries < batch)
       return KM_ERROR_NOT_SUPPORTED; // a plan needs every entry's workspace
     c.a.order = p->order;
+    km_status bs = bind_plan(c, r, stream);
+    if (bs != KM_SUCCESS)
+      return bs;
   }
   const int occ1 = occupancy(k1.fn, k1.nt);
   const int occ2 = occupancy(k2.fn, k2.nt);
@@ -375,6 +418,9 @@ km_status launch_twopass_fwd(Call<T> &c, const Resolved &r, const KernelInfo &k1
     if (chunk_entries < batch)
       return KM_ERROR_NOT_SUPPORTED; // a plan needs every entry's workspace
     c.a.order = p->order;
+    km_status bs = bind_plan(c, r, stream);
+    if (bs != KM_SUCCESS)
+      return bs;
   }
   const int occ1 = occupancy(k1.fn, k1.nt);
   const int occ2 = occupancy(k2.fn, k2.nt);
@@ -477,6 +523,9 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
     c.a.ranges = p->ranges;
     c.a.order = p->order;
     c.a.owned = p->owned;
+    km_status bs = bind_plan(c, r, stream);
+    if (bs != KM_SUCCESS)
+      return bs;
   }
   else
   {
@@ -715,6 +764,28 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
   if (o.precision != KM_PRECISION_FP64 || o.plan)
     return KM_ERROR_NOT_SUPPORTED;
   const bool exact = (o.flags & KM_FLAG_EXACT) != 0;
+  // owned launch modes are bound to the plan's outputs (see bind_plan): a
+  // check kernel on the caller's stream ahead of every launch (and of the
+  // fork), or the plan's recorded outputs with KM_FLAG_PLAN_OUTPUTS
+  const bool owned_mode =
+      (o.flags & (KM_FLAG_PERSISTENT | KM_FLAG_DETERMINISTIC)) != 0;
+  const int *bind_flag = nullptr;
+  unsigned bind_gen = 0;
+  if (owned_mode && plan->ybind)
+  {
+    if (o.flags & KM_FLAG_PLAN_OUTPUTS)
+      pY = reinterpret_cast<T *const *>(plan->ybind);
+    else
+    {
+      bind_gen = ++plan->gen;
+      if (bind_gen == 0) // the flag starts at 0
+        bind_gen = ++plan->gen;
+      KM_HIP(km::plan_check_launch(reinterpret_cast<const void *const *>(pY),
+                                   plan->ybind, plan->batch, plan->flag,
+                                   bind_gen, (hipStream_t)stream));
+      bind_flag = plan->flag;
+    }
+  }
   if (!(o.flags & KM_FLAG_PERSISTENT))
   {
     // one LDS-resident launch per size bin, each over its plan ranges
@@ -723,6 +794,8 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     a.lda = 0;
     a.pX = pX;
     a.pY = pY;
+    a.bind_flag = bind_flag;
+    a.bind_gen = bind_gen;
     long long xtotal = 0;
     for (const km_group_bin &bn : plan->bins)
       xtotal += (long long)bn.cnt * ipow(bn.n, plan->d);
@@ -810,6 +883,8 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
   a.ranges = plan->ranges;
   a.order = plan->order;
   a.owned = 1;
+  a.bind_flag = bind_flag;
+  a.bind_gen = bind_gen;
   const int *nv = plan->nvec;
   void *args[] = {&a, &nv};
   KM_HIP(hipLaunchKernel(k.fn, dim3(plan->ngroups), dim3(k.nt), args, 0,
@@ -902,6 +977,35 @@ km_status xbatched_tables(int d, int batch, const T *const *Apool,
   const int cus = device_cus();
   KM_HIP(km::tables_launch<T>(t, cus, (hipStream_t)stream));
   return KM_SUCCESS;
+}
+
+// Workgroup count of an output-ordered plan for a square (d, n) call: one
+// resident wave of the default kernel's workgroups (of pass 2 for the
+// two-pass family), at most one per entry.
+long long plan_groups(int d, int n, int elem_bytes, int batch)
+{
+  Resolved r;
+  KernelInfo k;
+  bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
+                                  : pick_onchip<float>(d, n, r, &k);
+  long long G;
+  KernelInfo k1, k2;
+  bool twopass = !onchip && ((elem_bytes == 8)
+                                 ? km::twopass_kernels<double>(d, n, false, &k1, &k2)
+                                 : km::twopass_kernels<float>(d, n, false, &k1, &k2));
+  if (!onchip && !twopass)
+    k = (elem_bytes == 8) ? km::global_kernel<double>(false)
+                          : km::global_kernel<float>(false);
+  if (twopass)
+  {
+    // pass 2 runs one workgroup per (entry group, block of positions)
+    const long long gy = (ipow(n, d - 2) + k2.nt - 1) / k2.nt;
+    G = std::max(1LL, (long long)device_cus() * occupancy(k2.fn, k2.nt) * 2 / gy);
+  }
+  else
+    G = (long long)device_cus() * occupancy(k.fn, k.nt);
+  G = std::min<long long>(G, km::kOrderMaxGroups);
+  return std::max(1LL, std::min<long long>(G, batch));
 }
 
 } // namespace
@@ -1033,84 +1137,7 @@ extern "C"
     if (s != KM_SUCCESS)
       return s;
     hipStream_t st = (hipStream_t)stream;
-    std::vector<uintptr_t> ys(batchCount), xs;
-    KM_HIP(hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
-                          hipMemcpyDeviceToHost, st));
-    if (order_by == KM_ORDER_BY_INPUT)
-    {
-      xs.resize(batchCount);
-      KM_HIP(hipMemcpyAsync(xs.data(), pX, sizeof(void *) * batchCount,
-                            hipMemcpyDeviceToHost, st));
-    }
-    KM_HIP(hipStreamSynchronize(st));
-    std::vector<int> order(batchCount);
-    std::iota(order.begin(), order.end(), 0);
-    if (order_by == KM_ORDER_BY_INPUT)
-    {
-      // entries sharing an input become neighbours (the repeated X reads hit
-      // L2); ties keep output order so equal outputs still merge on chip
-      std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return xs[x] != xs[y] ? xs[x] < xs[y] : ys[x] < ys[y];
-      });
-    }
-    else
-      std::stable_sort(order.begin(), order.end(),
-                       [&](int x, int y) { return ys[x] < ys[y]; });
-    // workgroup count the default kernel would use
-    Resolved r;
-    KernelInfo k;
-    bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
-                                    : pick_onchip<float>(d, n, r, &k);
-    long long G;
-    KernelInfo k1, k2;
-    bool twopass = !onchip && ((elem_bytes == 8)
-                                   ? km::twopass_kernels<double>(d, n, false, &k1, &k2)
-                                   : km::twopass_kernels<float>(d, n, false, &k1, &k2));
-    if (!onchip && !twopass)
-      k = (elem_bytes == 8) ? km::global_kernel<double>(false)
-                            : km::global_kernel<float>(false);
-    if (twopass)
-    {
-      // pass 2 runs one workgroup per (entry group, block of positions)
-      const long long gy = (ipow(n, d - 2) + k2.nt - 1) / k2.nt;
-      G = std::max(1LL, (long long)device_cus() * occupancy(k2.fn, k2.nt) * 2 / gy);
-    }
-    else
-      G = (long long)device_cus() * occupancy(k.fn, k.nt);
-    G = std::max(1LL, std::min<long long>(G, batchCount));
-    const long long target = (batchCount + G - 1) / G;
-    std::vector<int> ranges;
-    ranges.push_back(0);
-    int noutputs = 0;
-    if (order_by == KM_ORDER_BY_INPUT)
-    {
-      // outputs are shared between workgroups (atomic flush): even split
-      std::vector<uintptr_t> u(ys);
-      std::sort(u.begin(), u.end());
-      noutputs = (int)(std::unique(u.begin(), u.end()) - u.begin());
-      for (long long i = target; i < batchCount; i += target)
-        ranges.push_back((int)i);
-      ranges.push_back(batchCount);
-    }
-    else
-    {
-      // ranges end only where one output's run of entries ends
-      std::vector<int> starts;
-      std::vector<double> pre;
-      for (int i = 0; i < batchCount;)
-      {
-        int j = i + 1;
-        while (j < batchCount && ys[order[j]] == ys[order[i]])
-          j++;
-        starts.push_back(i);
-        pre.push_back((double)i);
-        i = j;
-      }
-      noutputs = (int)starts.size();
-      starts.push_back(batchCount);
-      pre.push_back((double)batchCount);
-      balanced_cuts(starts, pre, G, ranges);
-    }
+    const long long G = plan_groups(d, n, elem_bytes, batchCount);
     km_plan p = new km_plan_s;
     p->owned = (order_by == KM_ORDER_BY_OUTPUT) ? 1 : 0;
     p->d = d;
@@ -1118,15 +1145,117 @@ extern "C"
     p->elem_bytes = elem_bytes;
     p->batch = batchCount;
     (void)hipGetDevice(&p->device);
-    p->ngroups = (int)ranges.size() - 1;
-    p->noutputs = noutputs;
     km_status st2 = [&]() -> km_status {
-      KM_HIP(hipMalloc(&p->order, sizeof(int) * batchCount));
-      KM_HIP(hipMalloc(&p->ranges, sizeof(int) * ranges.size()));
-      KM_HIP(hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
-                       hipMemcpyHostToDevice));
-      KM_HIP(hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
-                       hipMemcpyHostToDevice));
+      if (order_by == KM_ORDER_BY_OUTPUT && batchCount <= km::kOrderMaxBatch)
+      {
+        // built on the device: stable radix sort by output, run starts and
+        // balanced cuts (km_order.hip); only the output count comes back
+        void *tmp = nullptr;
+        KM_HIP(hipMalloc(&tmp, km::order_bytes(batchCount, (int)G)));
+        km::OrderOut o{};
+        hipError_t e = km::order_build(pY, batchCount, (int)G, tmp, st, &o);
+        int nout[2] = {0, 0};
+        if (e == hipSuccess)
+          e = hipMalloc(&p->order, sizeof(int) * batchCount);
+        if (e == hipSuccess)
+          e = hipMalloc(&p->ranges, sizeof(int) * (G + 1));
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(p->order, o.order, sizeof(int) * batchCount,
+                             hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(p->ranges, o.ranges, sizeof(int) * (G + 1),
+                             hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(nout, o.nout, sizeof(nout), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+          e = hipStreamSynchronize(st);
+        (void)hipFree(tmp);
+        KM_HIP(e);
+        // launches cover the non-empty ranges only
+        p->ngroups = nout[1];
+        p->noutputs = nout[0];
+      }
+      else
+      {
+        // host-built order: input-ordered plans, and batches beyond the
+        // device builder's limit
+        std::vector<uintptr_t> ys(batchCount), xs;
+        KM_HIP(hipMemcpyAsync(ys.data(), pY, sizeof(void *) * batchCount,
+                              hipMemcpyDeviceToHost, st));
+        if (order_by == KM_ORDER_BY_INPUT)
+        {
+          xs.resize(batchCount);
+          KM_HIP(hipMemcpyAsync(xs.data(), pX, sizeof(void *) * batchCount,
+                                hipMemcpyDeviceToHost, st));
+        }
+        KM_HIP(hipStreamSynchronize(st));
+        std::vector<int> order(batchCount);
+        std::iota(order.begin(), order.end(), 0);
+        if (order_by == KM_ORDER_BY_INPUT)
+        {
+          // entries sharing an input become neighbours (the repeated X reads
+          // hit L2); ties keep output order so equal outputs still merge on
+          // chip
+          std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return xs[x] != xs[y] ? xs[x] < xs[y] : ys[x] < ys[y];
+          });
+        }
+        else
+          std::stable_sort(order.begin(), order.end(),
+                           [&](int x, int y) { return ys[x] < ys[y]; });
+        const long long target = (batchCount + G - 1) / G;
+        std::vector<int> ranges;
+        ranges.push_back(0);
+        int noutputs = 0;
+        if (order_by == KM_ORDER_BY_INPUT)
+        {
+          // outputs are shared between workgroups (atomic flush): even split
+          std::vector<uintptr_t> u(ys);
+          std::sort(u.begin(), u.end());
+          noutputs = (int)(std::unique(u.begin(), u.end()) - u.begin());
+          for (long long i = target; i < batchCount; i += target)
+            ranges.push_back((int)i);
+          ranges.push_back(batchCount);
+        }
+        else
+        {
+          // ranges end only where one output's run of entries ends
+          std::vector<int> starts;
+          std::vector<double> pre;
+          for (int i = 0; i < batchCount;)
+          {
+            int j = i + 1;
+            while (j < batchCount && ys[order[j]] == ys[order[i]])
+              j++;
+            starts.push_back(i);
+            pre.push_back((double)i);
+            i = j;
+          }
+          noutputs = (int)starts.size();
+          starts.push_back(batchCount);
+          pre.push_back((double)batchCount);
+          balanced_cuts(starts, pre, G, ranges);
+        }
+        p->ngroups = (int)ranges.size() - 1;
+        p->noutputs = noutputs;
+        KM_HIP(hipMalloc(&p->order, sizeof(int) * batchCount));
+        KM_HIP(hipMalloc(&p->ranges, sizeof(int) * ranges.size()));
+        KM_HIP(hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
+                         hipMemcpyHostToDevice));
+        KM_HIP(hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
+                         hipMemcpyHostToDevice));
+      }
+      if (p->owned)
+      {
+        // the binding: a device copy of the output pointers the plan was
+        // built from, and the check kernel's flag
+        KM_HIP(hipMalloc(&p->ybind, sizeof(void *) * batchCount));
+        KM_HIP(hipMalloc(&p->flag, sizeof(int)));
+        KM_HIP(hipMemcpyAsync(p->ybind, pY, sizeof(void *) * batchCount,
+                              hipMemcpyDeviceToDevice, st));
+        KM_HIP(hipMemsetAsync(p->flag, 0, sizeof(int), st));
+        KM_HIP(hipStreamSynchronize(st));
+      }
       return KM_SUCCESS;
     }();
     if (st2 != KM_SUCCESS)
@@ -1146,6 +1275,10 @@ extern "C"
       (void)hipFree(plan->order);
     if (plan->ranges)
       (void)hipFree(plan->ranges);
+    if (plan->ybind)
+      (void)hipFree(plan->ybind);
+    if (plan->flag)
+      (void)hipFree(plan->flag);
     delete plan;
     return KM_SUCCESS;
   }
@@ -1358,6 +1491,12 @@ extern "C"
                        hipMemcpyHostToDevice));
       KM_HIP(hipMemcpy(p->ranges, ranges.data(), sizeof(int) * ranges.size(),
                        hipMemcpyHostToDevice));
+      // the binding of the owned launch modes (as for km_plan_create)
+      KM_HIP(hipMalloc(&p->ybind, sizeof(void *) * batchCount));
+      KM_HIP(hipMalloc(&p->flag, sizeof(int)));
+      KM_HIP(hipMemcpy(p->ybind, ys.data(), sizeof(void *) * batchCount,
+                       hipMemcpyHostToDevice));
+      KM_HIP(hipMemset(p->flag, 0, sizeof(int)));
       // different sizes never share an output, so the size bins may run
       // concurrently: up to four streams, bins dealt out largest first
       // (used by the owned-output path)
@@ -1398,6 +1537,10 @@ extern "C"
       (void)hipFree(plan->ranges);
     if (plan->binranges)
       (void)hipFree(plan->binranges);
+    if (plan->ybind)
+      (void)hipFree(plan->ybind);
+    if (plan->flag)
+      (void)hipFree(plan->flag);
     for (hipStream_t st : plan->streams)
       (void)hipStreamDestroy(st);
     for (hipEvent_t ev : plan->joins)

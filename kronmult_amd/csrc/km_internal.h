@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stddef.h>
+
 #include "km_device.h"
 
 namespace km
@@ -120,6 +122,31 @@ bool grouped_kernel(int d, bool exact, KernelInfo *out, int *nmax);
 // kron_global (always available)
 template<typename T>
 KernelInfo global_kernel(bool exact);
+
+// ---------------------------------------------------------------------------
+// device-built output order (km_order.hip)
+// ---------------------------------------------------------------------------
+// the builder sorts in workgroups of 4096 entries; at most this many
+constexpr int kOrderMaxBlocks = 1024;
+constexpr long long kOrderMaxBatch = 4096LL * kOrderMaxBlocks;
+// and cut the order into at most this many workgroup ranges
+constexpr int kOrderMaxGroups = 8192;
+
+struct OrderOut
+{
+  int *order;  // batch entry indices, grouped by output, entry order kept
+  int *ranges; // G + 1 range starts: the non-empty ranges first, then
+               // empty ones at the end of the order
+  int *nout;   // [0] number of distinct outputs, [1] non-empty ranges
+};
+// bytes of device buffer order_build needs
+size_t order_bytes(long long batch, int G);
+// build the order of pY into buf (device memory of order_bytes), async on st
+hipError_t order_build(const void *const *pY, int batch, int G, void *buf,
+                       hipStream_t st, OrderOut *out);
+// *flag = gen when pY differs anywhere from ybind (batch pointers), async
+hipError_t plan_check_launch(const void *const *pY, const void *const *ybind,
+                             int batch, int *flag, unsigned gen, hipStream_t st);
 
 } // namespace km
 

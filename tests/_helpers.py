@@ -70,7 +70,11 @@ def _torch():
     return torch
 
 
-def gpu_pbatched(A, X, Y0, yidx, xidx=None, opt=None, with_w=False, check_x=True):
+def gpu_pbatched(A, X, Y0, yidx, xidx=None, opt=None, with_w=False, check_x=True,
+                 plan_kw=None, calls=1):
+    """pbatched through the C ABI.  plan_kw (a dict of km.options keywords)
+    builds an output-ordered plan from this call's own pY and passes it;
+    calls > 1 repeats the call from Y0 and returns every result."""
     import kronmult_amd as km
     torch = _torch()
     dev = torch.device("cuda:0")
@@ -79,18 +83,31 @@ def gpu_pbatched(A, X, Y0, yidx, xidx=None, opt=None, with_w=False, check_x=True
     xidx = np.arange(batch) if xidx is None else np.asarray(xidx)
     tA = torch.tensor(A, device=dev)
     tX = torch.tensor(X, device=dev)
-    tY = torch.tensor(Y0, device=dev)
+    tY0 = torch.tensor(Y0, device=dev)
+    tY = tY0.clone()
     pX = km.pointer_array(tX, xidx)
     pY = km.pointer_array(tY, yidx)
     tW = pW = None
     if with_w:
         tW = torch.zeros((batch, n ** d), dtype=tX.dtype, device=dev)
         pW = km.pointer_array(tW, np.arange(batch))
-    km.kronmult_pbatched(d, n, tA, pX, pY, pW, batch, dtype=dt, opt=opt)
-    torch.cuda.synchronize()
+    plan = None
+    if plan_kw is not None:
+        plan = km.Plan(d, n, X.itemsize, pY, batch)
+        opt = km.options(plan=plan, **plan_kw)
+    outs = []
+    try:
+        for _ in range(calls):
+            tY.copy_(tY0)
+            km.kronmult_pbatched(d, n, tA, pX, pY, pW, batch, dtype=dt, opt=opt)
+            torch.cuda.synchronize()
+            outs.append(tY.cpu().numpy())
+    finally:
+        if plan is not None:
+            plan.close()
     if check_x:
         assert np.array_equal(tX.cpu().numpy(), X), "X was modified"
-    return tY.cpu().numpy()
+    return outs[0] if calls == 1 else outs
 
 
 def gpu_batched(A, X, Y0, opt=None, with_w=False):

@@ -108,14 +108,10 @@ def test_exact_forward_slab(km, d, n, mode):
     A, X, Y0 = rng_inputs(820 + d * n, d, n, batch, batch)
     if mode == "plan":
         yidx = np.array([0, 0, 1, 1, 0, 1])
-        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
-        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
-        opt = km.options(exact=True, variant=km.VARIANT_FORWARD, plan=plan)
+        got = gpu_pbatched(A, X, Y0, yidx, plan_kw=dict(exact=True, variant=km.VARIANT_FORWARD))
     else:
         yidx = np.arange(batch)
-        plan = None
-        opt = opt0
-    got = gpu_pbatched(A, X, Y0, yidx, opt=opt)
+        got = gpu_pbatched(A, X, Y0, yidx, opt=opt0)
     # one oracle thread: entries reach a shared output in entry order
     threads = O.num_threads()
     O.set_num_threads(1)
@@ -124,8 +120,6 @@ def test_exact_forward_slab(km, d, n, mode):
     finally:
         O.set_num_threads(threads)
     assert np.array_equal(got, want), rel_err(got, want)
-    if plan is not None:
-        plan.close()
 
 
 @pytest.mark.parametrize("d,n", [(6, 7), (6, 8)])
@@ -143,14 +137,11 @@ def test_exact_forward_twopass(km, d, n, mode):
     A, X, Y0 = rng_inputs(840 + d * n, d, n, batch, batch)
     if mode == "plan":
         yidx = np.array([0, 0, 1, 1, 0, 1])[:batch]
-        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
-        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
-        opt = km.options(exact=True, variant=km.VARIANT_FORWARD, plan=plan)
+        got = gpu_pbatched(A, X, Y0, yidx, with_w=True,
+                           plan_kw=dict(exact=True, variant=km.VARIANT_FORWARD))
     else:
         yidx = np.arange(batch)
-        plan = None
-        opt = opt0
-    got = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
+        got = gpu_pbatched(A, X, Y0, yidx, opt=opt0, with_w=True)
     threads = O.num_threads()
     O.set_num_threads(1)
     try:
@@ -158,8 +149,6 @@ def test_exact_forward_twopass(km, d, n, mode):
     finally:
         O.set_num_threads(threads)
     assert np.array_equal(got, want), rel_err(got, want)
-    if plan is not None:
-        plan.close()
 
 
 @pytest.mark.parametrize("d,n", [(2, 3), (3, 4), (4, 5), (6, 3), (6, 4), (6, 6)])
@@ -330,14 +319,12 @@ def test_plan_deterministic_and_correct(km, d, n, pattern):
     plan = km.Plan(d, n, 8, pY, batch)
     nout, ngroups = plan.info()
     assert nout == len(set(yidx.tolist()))
-    opt = km.options(plan=plan)
-    got1 = gpu_pbatched(A, X, Y0, yidx, opt=opt)
-    got2 = gpu_pbatched(A, X, Y0, yidx, opt=opt)
+    plan.close()
+    del tY, pY
+    got1, got2 = gpu_pbatched(A, X, Y0, yidx, plan_kw={}, calls=2)
     assert np.array_equal(got1, got2), "plan mode must be bitwise reproducible"
     want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
     assert rel_err(got1, want) <= RTOL_FP64
-    plan.close()
-    del tY, pY
 
 
 def test_plan_mismatch_rejected(km):
@@ -394,17 +381,9 @@ def test_streamed_inputs_vs_oracle(km, d, n, batch, mode):
     A, X, Y0 = rng_inputs(950 + d * n, d, n, batch, batch // 4)
     assert X.nbytes >= 64 << 20
     yidx = np.arange(batch) // 4
-    opt = None
-    plan = None
-    if mode == "plan":
-        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
-        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
-        opt = km.options(plan=plan)
-    got = gpu_pbatched(A, X, Y0, yidx, opt=opt)
+    got = gpu_pbatched(A, X, Y0, yidx, plan_kw={} if mode == "plan" else None)
     want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
     assert rel_err(got, want) <= RTOL_FP64
-    if plan is not None:
-        plan.close()
 
 
 @pytest.mark.parametrize("mode", ["atomic", "plan"])
@@ -533,17 +512,9 @@ def test_twopass_merged_runs_integer_bitwise(km, mode):
     d, n, batch = 6, 7, 12
     A, X, Y0 = int_inputs(81, d, n, batch, 4)
     yidx = np.arange(batch) // 3
-    opt = None
-    plan = None
-    if mode == "plan":
-        tY = torch.tensor(Y0, device=torch.device("cuda:0"))
-        plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
-        opt = km.options(plan=plan)
-    got = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
+    got = gpu_pbatched(A, X, Y0, yidx, with_w=True, plan_kw={} if mode == "plan" else None)
     want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
     assert np.array_equal(got, want)
-    if plan is not None:
-        plan.close()
 
 
 def test_twopass_plan_deterministic(km):
@@ -551,16 +522,10 @@ def test_twopass_plan_deterministic(km):
     d, n, batch = 6, 8, 6
     A, X, Y0 = rng_inputs(79, d, n, batch, 2)
     yidx = np.array([0, 0, 1, 1, 0, 1])
-    dev = torch.device("cuda:0")
-    tY = torch.tensor(Y0, device=dev)
-    plan = km.Plan(d, n, 8, km.pointer_array(tY, yidx), batch)
-    opt = km.options(plan=plan)
-    g1 = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
-    g2 = gpu_pbatched(A, X, Y0, yidx, opt=opt, with_w=True)
+    g1, g2 = gpu_pbatched(A, X, Y0, yidx, with_w=True, plan_kw={}, calls=2)
     assert np.array_equal(g1, g2)
     want = O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD)
     assert rel_err(g1, want) <= RTOL_FP64
-    plan.close()
 
 
 def test_input_ordered_plan_pooled_xbatched(km):

@@ -1,0 +1,179 @@
+"""Plans are bound to the output pointers they were built from.
+
+A plan built from pY1 and then given a different pY2 (outputs regrouped, or
+another output buffer) must still give the right answer for pY2: the call's
+check kernel sees the difference and the kernels accumulate with atomics
+instead of the plan's owned read-modify-writes, so no output is written by
+two workgroups without atomics.  With KM_FLAG_PLAN_OUTPUTS the call writes to
+the plan's recorded outputs (pY1) and ignores pY.
+
+Small-integer inputs keep every sum exact in any order, so every comparison
+here is bitwise; a lost or doubled update shows as a wrong integer.
+Covered: kron_lds (d4n5), kron_reg (d6n4), kron_slab (d6n6), the two-pass
+pair (d6n8), pooled xbatched with a plan, and the grouped owned modes."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _helpers import int_inputs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def km():
+    import torch
+    import kronmult_amd
+    assert torch.cuda.is_available()
+    kronmult_amd.lib()
+    return kronmult_amd
+
+
+def _dev():
+    import torch
+    return torch.device("cuda:0")
+
+
+@pytest.mark.parametrize("d,n,batch", [(4, 5, 96), (6, 4, 96), (6, 6, 24), (6, 8, 8)])
+def test_pbatched_plan_with_regrouped_outputs(km, d, n, batch):
+    import torch
+    dev = _dev()
+    nY = 4
+    A, X, Y0 = int_inputs(910 + d * n, d, n, batch, nY)
+    y1 = np.arange(batch) * nY // batch                 # blocks of equal outputs
+    y2 = np.random.default_rng(7).permutation(batch) % nY  # interleaved, shuffled
+    tA = torch.tensor(A, device=dev)
+    tX = torch.tensor(X, device=dev)
+    tY0 = torch.tensor(Y0, device=dev)
+    tY = tY0.clone()
+    pX = km.pointer_array(tX, np.arange(batch))
+    pY1 = km.pointer_array(tY, y1)
+    pY2 = km.pointer_array(tY, y2)
+    tW = torch.zeros((batch, n ** d), dtype=torch.float64, device=dev)
+    pW = km.pointer_array(tW, np.arange(batch))
+    plan = km.Plan(d, n, 8, pY1, batch)
+    try:
+        # the plan's own outputs
+        km.kronmult_pbatched(d, n, tA, pX, pY1, pW, batch, opt=km.options(plan=plan))
+        torch.cuda.synchronize()
+        assert np.array_equal(tY.cpu().numpy(), O.pbatched(A, X, Y0, y1, variant=O.BACKWARD))
+        # regrouped outputs: atomics, still exact
+        tY.copy_(tY0)
+        km.kronmult_pbatched(d, n, tA, pX, pY2, pW, batch, opt=km.options(plan=plan))
+        torch.cuda.synchronize()
+        assert np.array_equal(tY.cpu().numpy(), O.pbatched(A, X, Y0, y2, variant=O.BACKWARD))
+        # a later call with the plan's own outputs is owned again
+        tY.copy_(tY0)
+        km.kronmult_pbatched(d, n, tA, pX, pY1, pW, batch, opt=km.options(plan=plan))
+        torch.cuda.synchronize()
+        assert np.array_equal(tY.cpu().numpy(), O.pbatched(A, X, Y0, y1, variant=O.BACKWARD))
+        # KM_FLAG_PLAN_OUTPUTS: the recorded outputs (y1), pY2 not read
+        tY.copy_(tY0)
+        km.kronmult_pbatched(d, n, tA, pX, pY2, pW, batch, opt=km.options(plan=plan, plan_outputs=True))
+        torch.cuda.synchronize()
+        assert np.array_equal(tY.cpu().numpy(), O.pbatched(A, X, Y0, y1, variant=O.BACKWARD))
+    finally:
+        plan.close()
+
+
+def test_pbatched_plan_on_another_output_buffer(km):
+    """Same grouping, but the outputs moved to a new buffer: the call must
+    accumulate into the new buffer and leave the old one alone."""
+    import torch
+    dev = _dev()
+    d, n, batch, nY = 6, 6, 16, 4
+    A, X, Y0 = int_inputs(930, d, n, batch, nY)
+    yidx = np.arange(batch) // 4
+    tA = torch.tensor(A, device=dev)
+    tX = torch.tensor(X, device=dev)
+    old = torch.tensor(Y0, device=dev)
+    new = torch.tensor(Y0, device=dev)
+    pX = km.pointer_array(tX, np.arange(batch))
+    plan = km.Plan(d, n, 8, km.pointer_array(old, yidx), batch)
+    try:
+        km.kronmult_pbatched(d, n, tA, pX, km.pointer_array(new, yidx), None, batch, opt=km.options(plan=plan))
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+    assert np.array_equal(old.cpu().numpy(), Y0)
+    assert np.array_equal(new.cpu().numpy(), O.pbatched(A, X, Y0, yidx, variant=O.BACKWARD))
+
+
+def test_xbatched_plan_with_regrouped_outputs(km):
+    import torch
+    dev = _dev()
+    d, n, batch, nY, P = 6, 4, 256, 16, 5
+    rng = np.random.default_rng(940)
+    pools = [rng.integers(-2, 3, (P, n, n)).astype(np.float64) for _ in range(d)]
+    aidx = rng.integers(0, P, (batch, d))
+    X = rng.integers(-2, 3, (batch, n ** d)).astype(np.float64)
+    Y0 = rng.integers(-4, 5, (nY, n ** d)).astype(np.float64)
+    y1 = np.arange(batch) // (batch // nY)
+    y2 = rng.integers(0, nY, batch)
+    xidx = rng.permutation(batch)
+    tpools = [torch.tensor(p, device=dev) for p in pools]
+    table = km.matrix_pointer_table(tpools, aidx)
+    tX = torch.tensor(X, device=dev)
+    tY0 = torch.tensor(Y0, device=dev)
+    tY = tY0.clone()
+    pX = km.pointer_array(tX, xidx)
+    pY1 = km.pointer_array(tY, y1)
+    pY2 = km.pointer_array(tY, y2)
+    plan = km.Plan(d, n, 8, pY1, batch)
+    try:
+        for pY, yy in ((pY2, y2), (pY1, y1)):
+            tY.copy_(tY0)
+            km.kronmult_xbatched(d, n, table, n, pX, pY, None, batch, opt=km.options(plan=plan))
+            torch.cuda.synchronize()
+            want = O.xbatched(pools, aidx, X, xidx, Y0, yy, variant=O.BACKWARD)
+            assert np.array_equal(tY.cpu().numpy(), want)
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("mode", ["deterministic", "persistent"])
+def test_grouped_owned_modes_with_regrouped_outputs(km, mode):
+    import torch
+    dev = _dev()
+    d, sizes, batch, groups = 3, (2, 4, 5), 240, 6
+    rng = np.random.default_rng(950)
+    ns = rng.choice(sizes, batch)
+    g1 = np.zeros(batch, dtype=np.int64)
+    g2 = np.zeros(batch, dtype=np.int64)
+    for n in sizes:
+        idx = np.flatnonzero(ns == n)
+        g1[idx] = np.arange(len(idx)) * groups // len(idx)   # blocks
+        g2[idx] = rng.integers(0, groups, len(idx))           # regrouped
+    mats = [[rng.integers(-2, 3, (n, n)).astype(np.float64) for _ in range(d)] for n in ns]
+    X = [rng.integers(-2, 3, n ** d).astype(np.float64) for n in ns]
+    Y0 = {n: rng.integers(-4, 5, (groups, n ** d)).astype(np.float64) for n in sizes}
+    tY = {n: torch.tensor(v, device=dev) for n, v in Y0.items()}
+    tX = [torch.tensor(x, device=dev) for x in X]
+    tA = [torch.tensor(np.ascontiguousarray(mats[b][k].T), device=dev) for b in range(batch) for k in range(d)]
+    table = torch.tensor([t.data_ptr() for t in tA], dtype=torch.int64, device=dev)
+    pX = torch.tensor([t.data_ptr() for t in tX], dtype=torch.int64, device=dev)
+
+    def ptrs(gg):
+        return torch.tensor([tY[n][gg[b]].data_ptr() for b, n in enumerate(ns)], dtype=torch.int64, device=dev)
+
+    def oracle(gg):
+        want = {n: v.copy() for n, v in Y0.items()}
+        for b, n in enumerate(ns):
+            want[n][gg[b]] += O.brute(mats[b], X[b])
+        return want
+
+    pY1, pY2 = ptrs(g1), ptrs(g2)
+    plan = km.GroupPlan(d, [int(v) for v in ns], pY1)
+    try:
+        for pY, gg in ((pY2, g2), (pY1, g1)):
+            for n in sizes:
+                tY[n].copy_(torch.tensor(Y0[n], device=dev))
+            km.kronmult_grouped(plan, table, pX, pY, opt=km.options(**{mode: True}))
+            torch.cuda.synchronize()
+            want = oracle(gg)
+            for n in sizes:
+                assert np.array_equal(tY[n].cpu().numpy(), want[n]), (mode, n)
+    finally:
+        plan.close()
