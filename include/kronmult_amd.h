@@ -208,6 +208,31 @@ km_status km_plan_create_ordered(km_plan *plan, int d, int n, int elem_bytes,
                                  const void *const *pX, const void *const *pY,
                                  int batchCount, int order_by, void *stream);
 km_status km_plan_destroy(km_plan plan);
+
+/* ---------------------------------------------------------------------------
+ * Allocation-free deterministic pbatched.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Computes
+ * what km_plan_create + km_kronmult_pbatched_f64 with that plan +
+ * km_plan_destroy compute, with the same kernels and bitwise the same
+ * results, but allocates nothing: the output-ordered entry order is built on
+ * the device, on `stream`, in a static buffer of 8 MiB that each device the
+ * library runs on holds until the process ends.
+ *   - batchCount <= KM_NOALLOC_MAX_BATCH; larger batches return
+ *     KM_ERROR_INVALID_VALUE (km_last_error says why);
+ *   - opt->plan must be NULL (the call builds its own order);
+ *   - every call on one device shares that device's buffer: calls that
+ *     overlap on one device (two streams, two host threads) race and must be
+ *     serialized by the caller;
+ *   - pW as for km_kronmult_pbatched_f64 (d = 6 with n >= 7 needs it).
+ * ------------------------------------------------------------------------ */
+#define KM_NOALLOC_MAX_BATCH 262144
+km_status km_kronmult_pbatched_noalloc_f64(int d, int n, const double *A,
+                                           double *const *pX,
+                                           double *const *pY,
+                                           double *const *pW, int batchCount,
+                                           void *stream,
+                                           const km_options *opt);
 /* number of distinct outputs and of workgroups in the plan */
 km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
 

@@ -65,7 +65,11 @@ C_ABI_SYMBOLS = (
     "km_kgemm_nn_batched_f64", "km_kgemm_nt_batched_f64", "km_kgemm_tt_batched_f64",
     "km_kgemm_nn_batched_f32", "km_kgemm_nt_batched_f32", "km_kgemm_tt_batched_f32",
     "km_xbatched_tables_f64", "km_xbatched_tables_f32",
+    "km_kronmult_pbatched_noalloc_f64",
 )
+
+# largest batchCount of kronmult_pbatched_noalloc (KM_NOALLOC_MAX_BATCH)
+NOALLOC_MAX_BATCH = 262144
 
 
 class KronmultError(RuntimeError):
@@ -136,6 +140,8 @@ def lib():
             f = getattr(L, f"km_kronmult_vbatched_{suf}")
             f.argtypes = [_I, _P, _P, _P, _P, _P, _P, _SZ, _I, _P, C.POINTER(Options)]
             f.restype = _I
+        L.km_kronmult_pbatched_noalloc_f64.argtypes = [_I, _I, _P, _P, _P, _P, _I, _P, C.POINTER(Options)]
+        L.km_kronmult_pbatched_noalloc_f64.restype = _I
         L.km_plan_create.argtypes = [C.POINTER(C.c_void_p), _I, _I, _I, _P, _I, _P]
         L.km_plan_create.restype = _I
         L.km_plan_create_ordered.argtypes = [C.POINTER(C.c_void_p), _I, _I, _I, _P, _P, _I, _I, _P]
@@ -254,6 +260,17 @@ def kronmult_pbatched(d: int, n: int, A, pX, pY, pW, batchCount: int, dtype="f64
     f = getattr(lib(), f"km_kronmult_pbatched_{dtype}")
     _check(f(d, n, _addr(A), _addr(pX), _addr(pY), _addr(pW), batchCount, _stream(stream), _opt_ref(opt)),
            f"kronmult{d}_pbatched")
+
+
+def kronmult_pbatched_noalloc(d: int, n: int, A, pX, pY, pW, batchCount: int, stream=None,
+                              opt: Optional[Options] = None) -> None:
+    """Deterministic (output-ordered) FP64 pbatched that allocates nothing:
+    the entry order is built on the device in the library's static
+    per-device buffer (km_kronmult_pbatched_noalloc_f64).  batchCount <=
+    NOALLOC_MAX_BATCH; calls that overlap on one device race."""
+    _check(lib().km_kronmult_pbatched_noalloc_f64(d, n, _addr(A), _addr(pX), _addr(pY), _addr(pW), batchCount,
+                                                  _stream(stream), _opt_ref(opt)),
+           f"kronmult{d}_pbatched_noalloc")
 
 
 def kronmult_xbatched(d: int, n: int, Aarray, lda: int, pX, pY, pW, batchCount: int,

@@ -979,6 +979,14 @@ km_status xbatched_tables(int d, int batch, const T *const *Apool,
   return KM_SUCCESS;
 }
 
+// km_kronmult_pbatched_noalloc_f64: pbatched with an output-ordered plan that
+// lives on the stack and points into the static per-device buffer, where the
+// order is built on the device; no allocation, no host synchronization.
+template<typename T>
+km_status pbatched_noalloc(int d, int n, const T *A, T *const *pX,
+                           T *const *pY, T *const *pW, int batch, void *stream,
+                           const km_options *opt);
+
 // Workgroup count of an output-ordered plan for a square (d, n) call: one
 // resident wave of the default kernel's workgroups (of pass 2 for the
 // two-pass family), at most one per entry.
@@ -1006,6 +1014,63 @@ long long plan_groups(int d, int n, int elem_bytes, int batch)
     G = (long long)device_cus() * occupancy(k.fn, k.nt);
   G = std::min<long long>(G, km::kOrderMaxGroups);
   return std::max(1LL, std::min<long long>(G, batch));
+}
+
+template<typename T>
+km_status pbatched_noalloc(int d, int n, const T *A, T *const *pX,
+                           T *const *pY, T *const *pW, int batch, void *stream,
+                           const km_options *opt)
+{
+  km_status s = check_dn(d, n, batch);
+  if (s != KM_SUCCESS)
+    return s;
+  if (opt && opt->plan)
+    return KM_ERROR_INVALID_VALUE;
+  if (batch > KM_NOALLOC_MAX_BATCH)
+  {
+    std::snprintf(g_last_error, sizeof(g_last_error),
+                  "km_kronmult_pbatched_noalloc: batchCount %d exceeds "
+                  "KM_NOALLOC_MAX_BATCH (%d)",
+                  batch, KM_NOALLOC_MAX_BATCH);
+    return KM_ERROR_INVALID_VALUE;
+  }
+  if (batch == 0)
+    return KM_SUCCESS;
+  if (!A || !pX || !pY)
+    return KM_ERROR_INVALID_VALUE;
+  const int G = (int)plan_groups(d, n, sizeof(T), batch);
+  if (km::order_bytes(batch, G) > KM_NOALLOC_SCRATCH_BYTES)
+  {
+    std::snprintf(g_last_error, sizeof(g_last_error),
+                  "km_kronmult_pbatched_noalloc: order of %d entries in %d "
+                  "ranges exceeds the static buffer",
+                  batch, G);
+    return KM_ERROR_INVALID_VALUE;
+  }
+  void *buf = nullptr;
+  KM_HIP(km::noalloc_scratch(&buf));
+  km::OrderOut o{};
+  KM_HIP(km::order_build(reinterpret_cast<const void *const *>(pY), batch, G,
+                         buf, (hipStream_t)stream, &o));
+  // a plan for this call only: G workgroups, the empty ranges at the end
+  // exit at once; no binding copy (the order was built from this pY)
+  km_plan_s p;
+  p.d = d;
+  p.n = n;
+  p.elem_bytes = (int)sizeof(T);
+  p.batch = batch;
+  KM_HIP(hipGetDevice(&p.device));
+  p.ngroups = G;
+  p.noutputs = 0;
+  p.owned = 1;
+  p.order = o.order;
+  p.ranges = o.ranges;
+  km_options o2;
+  km_options_init(&o2);
+  if (opt)
+    o2 = *opt;
+  o2.plan = &p;
+  return pbatched<T>(d, n, A, pX, pY, pW, batch, stream, &o2);
 }
 
 } // namespace
@@ -1265,6 +1330,16 @@ extern "C"
     }
     *plan = p;
     return KM_SUCCESS;
+  }
+
+  km_status km_kronmult_pbatched_noalloc_f64(int d, int n, const double *A,
+                                             double *const *pX,
+                                             double *const *pY,
+                                             double *const *pW, int batchCount,
+                                             void *stream,
+                                             const km_options *opt)
+  {
+    return pbatched_noalloc<double>(d, n, A, pX, pY, pW, batchCount, stream, opt);
   }
 
   km_status km_plan_destroy(km_plan plan)

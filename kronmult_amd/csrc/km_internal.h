@@ -7,6 +7,12 @@
 #include <stddef.h>
 
 #include "km_device.h"
+#include "../../include/kronmult_amd.h"
+
+// size of the allocation-free entry point's static per-device buffer
+// (km_noalloc.hip): the device-built order of KM_NOALLOC_MAX_BATCH entries
+// cut into up to kOrderMaxGroups ranges needs 7.1 MiB
+#define KM_NOALLOC_SCRATCH_BYTES (8u << 20)
 
 namespace km
 {
@@ -147,6 +153,9 @@ hipError_t order_build(const void *const *pY, int batch, int G, void *buf,
 // *flag = gen when pY differs anywhere from ybind (batch pointers), async
 hipError_t plan_check_launch(const void *const *pY, const void *const *ybind,
                              int batch, int *flag, unsigned gen, hipStream_t st);
+// device address of the allocation-free entry point's static buffer on the
+// current device (KM_NOALLOC_SCRATCH_BYTES bytes; km_noalloc.hip)
+hipError_t noalloc_scratch(void **ptr);
 
 } // namespace km
 

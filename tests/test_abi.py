@@ -35,6 +35,13 @@ def test_every_declared_function_is_exported():
     assert not missing, missing
 
 
+def test_python_mirror_noalloc_limit_matches_header():
+    import re
+    import kronmult_amd as km
+    hdr = open(os.path.join(ROOT, "include", "kronmult_amd.h")).read()
+    assert int(re.search(r"#define KM_NOALLOC_MAX_BATCH (\d+)", hdr).group(1)) == km.NOALLOC_MAX_BATCH
+
+
 def test_python_mirror_lists_the_abi():
     import kronmult_amd as km
     assert set(km.C_ABI_SYMBOLS) == set(_header_functions())
@@ -74,6 +81,17 @@ def test_invalid_arguments_rejected_on_host():
     o.precision = 0
     o.variant = 9
     assert f(3, 4, 1, 1, 1, None, 2, None, C.byref(o)) == km.ERROR_INVALID_VALUE
+    # the allocation-free entry point: its size limit and arguments are
+    # checked before any device work
+    na = L.km_kronmult_pbatched_noalloc_f64
+    assert na(6, 4, 1, 1, 1, None, km.NOALLOC_MAX_BATCH + 1, None, None) == km.ERROR_INVALID_VALUE
+    assert b"KM_NOALLOC_MAX_BATCH" in L.km_last_error()
+    assert na(0, 4, 1, 1, 1, None, 4, None, None) == km.ERROR_INVALID_VALUE
+    assert na(3, 4, None, None, None, None, 4, None, None) == km.ERROR_INVALID_VALUE
+    assert na(3, 4, None, None, None, None, 0, None, None) == km.SUCCESS
+    o.variant = 0
+    o.plan = 1  # a plan is not accepted: the call builds its own order
+    assert na(3, 4, 1, 1, 1, None, 4, None, C.byref(o)) == km.ERROR_INVALID_VALUE
 
 
 def test_options_defaults_and_status_strings():
