@@ -282,6 +282,47 @@ km_status km_kronmult_grouped_f32(km_group_plan plan,
                                   const km_options *opt);
 
 /* ---------------------------------------------------------------------------
+ * Kernel-variant tuning.  Several kernels can serve one square (d, n) call
+ * with fused multiply-adds; the library keeps a table from (d, n, element
+ * size, owned outputs (an output-ordered plan) or not, batch-size bin
+ * floor(log2 batchCount)) to a variant, and uses its built-in choice where
+ * the table has no entry.  No counterpart in the reference (its launch
+ * geometry is fixed, kronmult6_batched.cpp:8-14).
+ *   km_tune_shape  times every candidate variant on synthetic device data of
+ *                  the given shape (allocates, fills and frees device memory,
+ *                  synchronizes `stream`; outputs shared `overlap`:1) and
+ *                  records the fastest in the table;
+ *   km_tune_set / km_tune_get  write / read one entry (KM_KERNEL_DEFAULT
+ *                  removes it);
+ *   km_tune_save / km_tune_load  the table as JSON, keyed by GPU architecture
+ *                  (a file tuned on another architecture is refused with
+ *                  KM_ERROR_NOT_SUPPORTED).  The file named by the environment
+ *                  variable KRONMULT_AMD_TUNE_FILE is loaded on first use.
+ * EXACT calls keep the built-in choice (it fixes the rounding order).
+ * ------------------------------------------------------------------------ */
+typedef enum km_kernel_variant
+{
+  KM_KERNEL_DEFAULT = 0,  /* built-in choice                                 */
+  KM_KERNEL_SLAB = 1,     /* slab-streamed, one workgroup per entry          */
+  KM_KERNEL_HALF1 = 2,    /* slab-streamed, two workgroups per entry split by
+                             the last mode's rows, one slab per step        */
+  KM_KERNEL_HALF2 = 3,    /* the same, two slabs per step                    */
+  KM_KERNEL_REG = 4,      /* register-blocked mode groups                    */
+  KM_KERNEL_LDS = 5,      /* whole tensor in LDS                             */
+  KM_KERNEL_LDS_WIDE = 6  /* whole tensor in LDS, wider workgroups           */
+} km_kernel_variant;
+
+km_status km_tune_shape(int d, int n, int elem_bytes, int batchCount,
+                        int overlap, int owned, void *stream, int *variant);
+km_status km_tune_set(int d, int n, int elem_bytes, int batchCount, int owned,
+                      int variant);
+km_status km_tune_get(int d, int n, int elem_bytes, int batchCount, int owned,
+                      int *variant);
+km_status km_tune_clear(void);
+km_status km_tune_save(const char *path);
+km_status km_tune_load(const char *path);
+
+/* ---------------------------------------------------------------------------
  * Introspection: which kernel family a square (d, n) call would use.
  *   *family: 0 = LDS-resident, 1 = slab-streamed, 2 = workspace-staged
  *            (generic), 3 = two-pass (sub-slabs in LDS, then the two
@@ -291,6 +332,12 @@ km_status km_kronmult_grouped_f32(km_group_plan plan,
  * ------------------------------------------------------------------------ */
 km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
                           int *family, int *threads);
+/* The same for a call of batchCount entries, with (owned = 1) or without an
+ * output-ordered plan: the tuned variant for that batch bin applies.
+ * *family 6 = split pair (two workgroups per entry). */
+km_status km_query_kernel_for(int d, int n, int elem_bytes, int batchCount,
+                              int owned, const km_options *opt, int *family,
+                              int *threads);
 
 /* Reference flop model (kron_flops, kroncommon.hpp:154-185) for one entry. */
 double km_kron_flops(int d, const int *m, const int *n);

@@ -50,6 +50,7 @@ FLAG_PLAN_OUTPUTS = 0x10
 
 # kernel families reported by query_kernel
 FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL, FAMILY_TWOPASS, FAMILY_REG = 0, 1, 2, 3, 4
+FAMILY_GROUPED, FAMILY_HALF = 5, 6
 
 # every symbol the C ABI exports (include/kronmult_amd.h)
 C_ABI_SYMBOLS = (
@@ -66,7 +67,12 @@ C_ABI_SYMBOLS = (
     "km_kgemm_nn_batched_f32", "km_kgemm_nt_batched_f32", "km_kgemm_tt_batched_f32",
     "km_xbatched_tables_f64", "km_xbatched_tables_f32",
     "km_kronmult_pbatched_noalloc_f64",
+    "km_tune_shape", "km_tune_set", "km_tune_get", "km_tune_clear", "km_tune_save", "km_tune_load",
+    "km_query_kernel_for",
 )
+
+# km_kernel_variant
+KERNEL_DEFAULT, KERNEL_SLAB, KERNEL_HALF1, KERNEL_HALF2, KERNEL_REG, KERNEL_LDS, KERNEL_LDS_WIDE = range(7)
 
 # largest batchCount of kronmult_pbatched_noalloc (KM_NOALLOC_MAX_BATCH)
 NOALLOC_MAX_BATCH = 262144
@@ -152,6 +158,16 @@ def lib():
         L.km_plan_info.restype = _I
         L.km_query_kernel.argtypes = [_I, _I, _I, C.POINTER(Options), C.POINTER(_I), C.POINTER(_I)]
         L.km_query_kernel.restype = _I
+        L.km_query_kernel_for.argtypes = [_I, _I, _I, _I, _I, C.POINTER(Options), C.POINTER(_I), C.POINTER(_I)]
+        L.km_query_kernel_for.restype = _I
+        L.km_tune_shape.argtypes = [_I, _I, _I, _I, _I, _I, _P, C.POINTER(_I)]
+        L.km_tune_set.argtypes = [_I, _I, _I, _I, _I, _I]
+        L.km_tune_get.argtypes = [_I, _I, _I, _I, _I, C.POINTER(_I)]
+        L.km_tune_clear.argtypes = []
+        L.km_tune_save.argtypes = [C.c_char_p]
+        L.km_tune_load.argtypes = [C.c_char_p]
+        for nm in ("km_tune_shape", "km_tune_set", "km_tune_get", "km_tune_clear", "km_tune_save", "km_tune_load"):
+            getattr(L, nm).restype = _I
         L.km_kron_flops.argtypes = [_I, _P, _P]
         L.km_kron_flops.restype = C.c_double
         L.km_group_plan_create.argtypes = [C.POINTER(C.c_void_p), _I, _I, _P, _P, _I, _P]
@@ -420,6 +436,48 @@ def query_kernel(d: int, n: int, elem_bytes: int = 8, opt: Optional[Options] = N
     _check(lib().km_query_kernel(d, n, elem_bytes, _opt_ref(opt), C.byref(fam), C.byref(nt)),
            "km_query_kernel")
     return fam.value, nt.value
+
+
+def query_kernel_for(d: int, n: int, batchCount: int, owned: bool = False, elem_bytes: int = 8,
+                     opt: Optional[Options] = None):
+    """(family, threads) of the kernel a call of batchCount entries uses, with
+    (owned) or without an output-ordered plan; tuned variants apply."""
+    fam, nt = C.c_int(), C.c_int()
+    _check(lib().km_query_kernel_for(d, n, elem_bytes, batchCount, int(owned), _opt_ref(opt),
+                                     C.byref(fam), C.byref(nt)), "km_query_kernel_for")
+    return fam.value, nt.value
+
+
+def tune_shape(d: int, n: int, batchCount: int, overlap: int = 1, owned: bool = False,
+               elem_bytes: int = 8, stream=None) -> int:
+    """Time every kernel variant for this shape on synthetic device data and
+    record the fastest (km_tune_shape); returns the chosen KERNEL_* value."""
+    v = C.c_int()
+    _check(lib().km_tune_shape(d, n, elem_bytes, batchCount, overlap, int(owned), _stream(stream),
+                               C.byref(v)), "km_tune_shape")
+    return v.value
+
+
+def tune_set(d: int, n: int, batchCount: int, variant: int, owned: bool = False, elem_bytes: int = 8) -> None:
+    _check(lib().km_tune_set(d, n, elem_bytes, batchCount, int(owned), variant), "km_tune_set")
+
+
+def tune_get(d: int, n: int, batchCount: int, owned: bool = False, elem_bytes: int = 8) -> int:
+    v = C.c_int()
+    _check(lib().km_tune_get(d, n, elem_bytes, batchCount, int(owned), C.byref(v)), "km_tune_get")
+    return v.value
+
+
+def tune_clear() -> None:
+    _check(lib().km_tune_clear(), "km_tune_clear")
+
+
+def tune_save(path: str) -> None:
+    _check(lib().km_tune_save(path.encode()), "km_tune_save")
+
+
+def tune_load(path: str) -> None:
+    _check(lib().km_tune_load(path.encode()), "km_tune_load")
 
 
 def kron_flops(m: Sequence[int], n: Sequence[int]) -> float:

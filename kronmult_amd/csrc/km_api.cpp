@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -200,6 +201,10 @@ bool alg993_forward(int d, const int *m, const int *n)
 
 struct Resolved
 {
+  // batch size and output ownership of the call, when known (selects the
+  // tuned kernel variant; 0 = the built-in choice)
+  long long batch = 0;
+  int owned = 0;
   unsigned flags = 0;
   bool exact = false;
   bool fwd = false;
@@ -236,6 +241,13 @@ bool pick_onchip(int d, int n, const Resolved &r, KernelInfo *k)
 {
   if (r.force_global)
     return false;
+  // FMA calls of a known batch size: the tuned variant, if any
+  if (!r.exact && r.batch > 0)
+  {
+    const int v = km::tuned_variant(d, n, (int)sizeof(T), r.owned, r.batch);
+    if (v != KM_KERNEL_DEFAULT && km::variant_kernel<T>(v, d, n, k))
+      return true;
+  }
   // without EXACT the contraction order only changes rounding, so the
   // forward request may use any kernel
   bool fwd = r.fwd && r.exact;
@@ -464,11 +476,14 @@ km_status launch_twopass_fwd(Call<T> &c, const Resolved &r, const KernelInfo &k1
 }
 
 template<typename T>
-km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
+km_status launch(Call<T> &c, const Resolved &r0, hipStream_t stream)
 {
   const int batch = c.a.batch;
   if (batch == 0)
     return KM_SUCCESS;
+  Resolved r = r0;
+  r.batch = batch;
+  r.owned = (r.plan && r.plan->owned) ? 1 : 0;
   KernelInfo k;
   bool onchip = c.square && pick_onchip<T>(c.sh.d, c.sh.n[0], r, &k);
   if (!onchip && c.square && !r.force_global && !(r.fwd && r.exact))
@@ -506,6 +521,8 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
     std::snprintf(g_last_error, sizeof(g_last_error), "no device / CU count");
     return KM_ERROR_HIP;
   }
+  // kron_half: two workgroups (a pair) per range of entries
+  const bool halves = onchip && k.family == km::FAM_HALF;
   int grid = 0;
   if (r.plan)
   {
@@ -530,6 +547,8 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
   else
   {
     long long g = (long long)cus * occupancy(k.fn, k.nt);
+    if (halves)
+      g = std::max(1LL, g / 2);
     if (!onchip)
       g = std::min(g, c.slots);
     g = std::max(1LL, std::min<long long>(g, batch));
@@ -541,7 +560,15 @@ km_status launch(Call<T> &c, const Resolved &r, hipStream_t stream)
     c.a.owned = 0;
   }
   hipError_t e;
-  if (onchip)
+  if (halves)
+  {
+    // pair q is served by workgroups 16*(q/8) + q%8 and that + 8
+    int npairs = grid;
+    const int g2 = (2 * npairs + 15) / 16 * 16;
+    void *args[] = {&c.a, &npairs};
+    e = hipLaunchKernel(k.fn, dim3(g2), dim3(k.nt), args, 0, stream);
+  }
+  else if (onchip)
   {
     void *args[] = {&c.a};
     e = hipLaunchKernel(k.fn, dim3(grid), dim3(k.nt), args, 0, stream);
@@ -993,6 +1020,8 @@ km_status pbatched_noalloc(int d, int n, const T *A, T *const *pX,
 long long plan_groups(int d, int n, int elem_bytes, int batch)
 {
   Resolved r;
+  r.batch = batch;
+  r.owned = 1;
   KernelInfo k;
   bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
                                   : pick_onchip<float>(d, n, r, &k);
@@ -1012,6 +1041,8 @@ long long plan_groups(int d, int n, int elem_bytes, int batch)
   }
   else
     G = (long long)device_cus() * occupancy(k.fn, k.nt);
+  if (onchip && k.family == km::FAM_HALF)
+    G = std::max(1LL, G / 2); // ranges are served by pairs of workgroups
   G = std::min<long long>(G, km::kOrderMaxGroups);
   return std::max(1LL, std::min<long long>(G, batch));
 }
@@ -1372,8 +1403,15 @@ extern "C"
   km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
                             int *family, int *threads)
   {
+    return km_query_kernel_for(d, n, elem_bytes, 0, 0, opt, family, threads);
+  }
+
+  km_status km_query_kernel_for(int d, int n, int elem_bytes, int batchCount,
+                                int owned, const km_options *opt, int *family,
+                                int *threads)
+  {
     km_status s = check_dn(d, n, 1);
-    if (s != KM_SUCCESS || (elem_bytes != 4 && elem_bytes != 8))
+    if (s != KM_SUCCESS || (elem_bytes != 4 && elem_bytes != 8) || batchCount < 0)
       return KM_ERROR_INVALID_VALUE;
     int m[6], nn[6];
     for (int k = 0; k < 6; k++)
@@ -1381,6 +1419,8 @@ extern "C"
     Resolved r;
     if ((s = resolve(opt, d, m, nn, &r)) != KM_SUCCESS)
       return s;
+    r.batch = batchCount;
+    r.owned = owned ? 1 : 0;
     KernelInfo k, k2;
     bool onchip = (elem_bytes == 8) ? pick_onchip<double>(d, n, r, &k)
                                     : pick_onchip<float>(d, n, r, &k);

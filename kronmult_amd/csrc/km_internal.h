@@ -24,7 +24,8 @@ enum Family
   FAM_GLOBAL = 2,
   FAM_TWOPASS = 3,
   FAM_REG = 4,
-  FAM_GROUPED = 5
+  FAM_GROUPED = 5,
+  FAM_HALF = 6
 };
 
 struct KernelInfo
@@ -108,6 +109,11 @@ bool reg_kernel(int d, int n, bool exact, KernelInfo *out);
 template<typename T>
 bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
 
+// kron_half (two workgroups per entry, split by the last mode's rows; FMA
+// calls) for (d, n) if one was built; pf = slabs per step (1 or 2)
+template<typename T>
+bool half_kernel(int d, int n, int pf, KernelInfo *out);
+
 // kron_slab_fwd (exact, forward mode order) for (d, n) if one was built
 template<typename T>
 bool slab_fwd_kernel(int d, int n, KernelInfo *out);
@@ -153,6 +159,17 @@ hipError_t order_build(const void *const *pY, int batch, int G, void *buf,
 // *flag = gen when pY differs anywhere from ybind (batch pointers), async
 hipError_t plan_check_launch(const void *const *pY, const void *const *ybind,
                              int batch, int *flag, unsigned gen, hipStream_t st);
+// ---------------------------------------------------------------------------
+// kernel-variant table (km_tune.cpp)
+// ---------------------------------------------------------------------------
+// floor(log2 batch): the batch-size bin of a tuned entry
+int batch_bin(long long batch);
+// variant (km_kernel_variant) recorded for an FMA call, KM_KERNEL_DEFAULT if none
+int tuned_variant(int d, int n, int elem_bytes, int owned, long long batch);
+// the kernel of variant v for square (d, n) FMA calls, if built
+template<typename T>
+bool variant_kernel(int v, int d, int n, KernelInfo *k);
+
 // device address of the allocation-free entry point's static buffer on the
 // current device (KM_NOALLOC_SCRATCH_BYTES bytes; km_noalloc.hip)
 hipError_t noalloc_scratch(void **ptr);

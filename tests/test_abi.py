@@ -164,3 +164,30 @@ def test_c_abi_header_is_plain_c(tmp_path):
     assert out.returncode == 0, out.stderr
     # defaults FP64 and backward order, the status text, the flop model
     assert out.stdout.strip() == f"0 0 invalid value {12 * 6 ** 7}"
+
+
+def test_tune_table_json_round_trip(tmp_path):
+    """The variant table's JSON file round trip (host-only: no GPU is needed
+    to set, save, clear and load entries)."""
+    import kronmult_amd as km
+    km.tune_clear()
+    try:
+        km.tune_set(6, 6, 4096, km.KERNEL_HALF2, owned=True)
+        km.tune_set(3, 8, 262144, km.KERNEL_LDS)
+        with pytest.raises(km.KronmultError):
+            km.tune_set(6, 6, 4096, km.KERNEL_REG)      # no kron_reg for d6n6
+        path = str(tmp_path / "t.json")
+        km.tune_save(path)
+        km.tune_clear()
+        assert km.tune_get(6, 6, 4096, owned=True) == km.KERNEL_DEFAULT
+        km.tune_load(path)
+        assert km.tune_get(6, 6, 4096, owned=True) == km.KERNEL_HALF2
+        assert km.tune_get(6, 6, 5000, owned=True) == km.KERNEL_HALF2   # same log2 bin
+        assert km.tune_get(6, 6, 8192, owned=True) == km.KERNEL_DEFAULT
+        assert km.tune_get(6, 6, 4096, owned=False) == km.KERNEL_DEFAULT
+        assert km.tune_get(3, 8, 262144) == km.KERNEL_LDS
+        import json
+        doc = json.load(open(path))
+        assert len(doc["entries"]) == 2
+    finally:
+        km.tune_clear()
