@@ -112,6 +112,83 @@ def cpu_baseline(A, X, yidx, n_y, overlap, reps=3):
     }
 
 
+PRECISIONS = {"fp64": 0, "fp64_mfma": 1, "f16x4": 2}
+
+
+def precision_study(dev, steps=10, warmup=2, nsample=16):
+    """The precision-study configuration: d=6, n=8, 1024 entries, outputs
+    shared 2:1, A ~ N(0,1)/sqrt(8) with one seeded row per matrix scaled by
+    1e3, X ~ U[-1,1], run with plain FP64 stages, FP64 matrix instructions and
+    the mixed FP16-slice stages (plan reduction).  Time per call, GFLOP/s and
+    the error against the FP64 CPU oracle on `nsample` sampled outputs."""
+    import numpy as np
+    import torch
+
+    import kronmult_amd as km
+    from oracle import oracle as O
+
+    d, n, batch, ov = 6, 8, 1024, 2
+    s = n ** d
+    g = torch.Generator(device=dev)
+    g.manual_seed(SEED + 4)
+    A = torch.randn((batch, d, n, n), generator=g, device=dev, dtype=torch.float64) / np.sqrt(8.0)
+    rows = torch.randint(0, n, (batch, d), generator=g, device=dev)
+    scale = torch.ones((batch, d, n), dtype=torch.float64, device=dev)
+    scale.scatter_(2, rows.unsqueeze(2), 1e3)
+    A *= scale.unsqueeze(2)  # row i of A_k is A[b, k, :, i]
+    X = torch.rand((batch, s), generator=g, device=dev, dtype=torch.float64) * 2 - 1
+    Y = torch.zeros((batch // ov, s), device=dev, dtype=torch.float64)
+    W = torch.empty((batch, s), device=dev, dtype=torch.float64)
+    yidx = np.arange(batch) // ov
+    pX = km.pointer_array(X, np.arange(batch))
+    pY = km.pointer_array(Y, yidx)
+    pW = km.pointer_array(W, np.arange(batch))
+    plan = km.Plan(d, n, 8, pY, batch)
+    outs = np.sort(np.random.default_rng(SEED).choice(batch // ov, nsample, replace=False))
+    ents = np.flatnonzero(np.isin(yidx, outs))
+    pos = {int(o): i for i, o in enumerate(outs)}
+    it = torch.as_tensor(ents, device=dev)
+    want = O.pbatched(A.index_select(0, it).cpu().numpy(), X.index_select(0, it).cpu().numpy(),
+                      np.zeros((len(outs), s)), np.array([pos[int(yidx[e])] for e in ents]),
+                      variant=O.BACKWARD)
+    scale_w = np.abs(want).max(axis=1, keepdims=True)
+    res = {"ms": {}, "gflops": {}, "speed_vs_fp64": {}, "rel_err_normwise": {},
+           "rel_err_max": {}, "rel_err_median": {}}
+    stream = torch.cuda.current_stream()
+    for name, prec in PRECISIONS.items():
+        opt = km.options(plan=plan, precision=prec)
+        Y.zero_()
+        km.kronmult_pbatched(d, n, A, pX, pY, pW, batch, stream=stream, opt=opt)
+        torch.cuda.synchronize()
+        got = Y.index_select(0, torch.as_tensor(outs, device=dev)).cpu().numpy()
+        el = (np.abs(got - want) / scale_w).max(axis=1)
+        res["rel_err_normwise"][name] = float(np.abs(got - want).max() / np.abs(want).max())
+        res["rel_err_max"][name] = float(el.max())
+        res["rel_err_median"][name] = float(np.median(el))
+        for _ in range(warmup):
+            km.kronmult_pbatched(d, n, A, pX, pY, pW, batch, stream=stream, opt=opt)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(steps):
+            km.kronmult_pbatched(d, n, A, pX, pY, pW, batch, stream=stream, opt=opt)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / steps
+        res["ms"][name] = round(ms, 4)
+        res["gflops"][name] = round(flops_per_call(d, n, batch) / (ms * 1e-3) / 1e9, 1)
+    for name in PRECISIONS:
+        res["speed_vs_fp64"][name] = round(res["ms"]["fp64"] / res["ms"][name], 3)
+    plan.close()
+    res["workload"] = ("kronmult6_pbatched d=6, n=8, batch=1024, 2:1 outputs, A ~ N(0,1)/sqrt(8) with one "
+                       "row per matrix x1e3, X ~ U[-1,1], plan reduction; fp64 = FP64 vector stages, "
+                       "fp64_mfma = FP64 matrix instructions, f16x4 = mixed FP16 slices (first pass; "
+                       "last two modes and Y in FP64)")
+    res["error_sample"] = (f"{len(outs)} outputs ({len(ents)} entries) recomputed by the FP64 CPU oracle; "
+                           "rel_err_max/median: over outputs of max|dY|/max|Y|")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +202,10 @@ def main():
                     help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["grouped", "xpool"], default="d6n6",
                     help="configuration (default d6n6: d=6, n=6, 4096 entries, 4:1)")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp64",
+                    help="stage arithmetic of the timed calls (fp64_mfma and f16x4: d6n8 only)")
+    ap.add_argument("--no-precision-study", action="store_true",
+                    help="skip the d=6, n=8 precision comparison added to the default line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -165,7 +246,9 @@ def main():
     # a plan is set-up work (once per output pointer array), outside the
     # timed region; every timed call does the full computation
     plan = km.Plan(d, n, 8, pY, batch) if args.mode in ("best", "plan") else None
-    modes = {"atomic": None, "plan": km.options(plan=plan) if plan else None}
+    prec = PRECISIONS[args.precision]
+    modes = {"atomic": km.options(precision=prec),
+             "plan": km.options(plan=plan, precision=prec) if plan else None}
     todo = ["atomic", "plan"] if args.mode == "best" else [args.mode]
 
     kernel_name = {0: "kron_lds", 1: "kron_slab", 2: "kron_global", 3: "kron_sub+kron_pair",
@@ -213,6 +296,11 @@ def main():
     if not args.no_cpu:
         # same seeded inputs, host copies
         cpu = cpu_baseline(A.cpu().numpy(), X.cpu().numpy(), yidx, n_y, overlap)
+    study = None
+    if args.workload == "d6n6" and not args.no_precision_study:
+        del A, X, Y, W
+        torch.cuda.empty_cache()
+        study = precision_study(dev)
 
     out = {
         "metric": "d=6 batched kronmult GFLOP/s + % HBM roofline; mixed vs FP64 at rel err <=1e-10",
@@ -225,7 +313,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp64_mfma": "f64", "f16x4": "f16 slices, f32 sums, f64 recombination"}[
+            args.precision],
         "data": "synthetic (seeded U[-1,1) A and X on device)",
         "config": {
             "workload": f"kronmult{d}_pbatched FP64, d={d}, n={n}, batch={batch}, {overlap}:1 output "
@@ -249,6 +338,10 @@ def main():
         "cpu_baseline": cpu,
         "wall_s": round(wall, 4),
     }
+    if args.precision != "fp64":
+        out["config"]["precision"] = args.precision
+    if study is not None:
+        out["precision_study"] = study
     print(json.dumps(out))
     if plan is not None:
         plan.close()
@@ -319,7 +412,9 @@ def bench_xpool(args, dev):
     pX = km.pointer_array(X, xidx)
     pY = km.pointer_array(Y, yidx)
     plan = km.Plan(d, n, 8, pY, batch) if args.mode in ("best", "plan") else None
-    modes = {"atomic": None, "plan": km.options(plan=plan) if plan else None}
+    prec = PRECISIONS[args.precision]
+    modes = {"atomic": km.options(precision=prec),
+             "plan": km.options(plan=plan, precision=prec) if plan else None}
     todo = ["atomic", "plan"] if args.mode == "best" else [args.mode]
     fam, nt = km.query_kernel(d, n, 8, None)
     kernel_name = {0: "kron_lds", 4: "kron_reg"}.get(fam, str(fam))
@@ -398,6 +493,10 @@ def bench_xpool(args, dev):
         "cpu_baseline": cpu,
         "wall_s": round(wall, 4),
     }
+    if args.precision != "fp64":
+        out["config"]["precision"] = args.precision
+    if study is not None:
+        out["precision_study"] = study
     print(json.dumps(out))
     if plan is not None:
         plan.close()
@@ -551,6 +650,10 @@ def bench_grouped(args, dev):
         "cpu_baseline": cpu,
         "wall_s": round(wall, 4),
     }
+    if args.precision != "fp64":
+        out["config"]["precision"] = args.precision
+    if study is not None:
+        out["precision_study"] = study
     print(json.dumps(out))
     plan.close()
 

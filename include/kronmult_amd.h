@@ -57,10 +57,22 @@ typedef enum km_variant
                                two, kronm.hpp:83-107 (square => backward)   */
 } km_variant;
 
-/* Arithmetic of the stage contractions. */
+/* Arithmetic of the stage contractions.  The two matrix-core values are
+ * built for the two-pass shapes at n = 8 (d = 6, double data, fused
+ * multiply-add calls); other calls with them return KM_ERROR_NOT_SUPPORTED.
+ * They change the first pass (modes d..3 of every sub-slab); the last two
+ * modes and the accumulation into Y stay FP64 vector arithmetic. */
 typedef enum km_precision
 {
-  KM_PRECISION_FP64 = 0      /* stages and accumulation in the data type     */
+  KM_PRECISION_FP64 = 0,      /* stages and accumulation in the data type    */
+  KM_PRECISION_FP64_MFMA = 1, /* FP64 matrix instructions for the stages     */
+  KM_PRECISION_F16X4 = 2      /* mixed: each operand split into four 10-bit
+                                 FP16 slices under a power-of-two scale (per
+                                 fiber of X, per row of A_k), FP16 matrix
+                                 instructions with exact FP32 sums, the slice
+                                 pairs recombined in FP64; about 40 bits kept
+                                 per stage (error ~1e-12 relative to the
+                                 largest element of the fiber and row)       */
 } km_precision;
 
 /* option flags */

@@ -40,6 +40,8 @@ VARIANT_ALG993 = 2
 
 # km_precision
 PRECISION_FP64 = 0
+PRECISION_FP64_MFMA = 1
+PRECISION_F16X4 = 2
 
 # flags
 FLAG_EXACT = 0x1

@@ -206,6 +206,7 @@ struct Resolved
   long long batch = 0;
   int owned = 0;
   unsigned flags = 0;
+  int precision = KM_PRECISION_FP64;
   bool exact = false;
   bool fwd = false;
   bool force_global = false;
@@ -219,8 +220,10 @@ km_status resolve(const km_options *opt, int d, const int *m, const int *n,
   km_options_init(&o);
   if (opt)
     o = *opt;
-  if (o.precision != KM_PRECISION_FP64)
+  if (o.precision != KM_PRECISION_FP64 && o.precision != KM_PRECISION_FP64_MFMA &&
+      o.precision != KM_PRECISION_F16X4)
     return KM_ERROR_NOT_SUPPORTED;
+  r->precision = o.precision;
   r->flags = o.flags;
   r->exact = (o.flags & KM_FLAG_EXACT) != 0;
   r->force_global = (o.flags & KM_FLAG_FORCE_GLOBAL) != 0;
@@ -484,6 +487,18 @@ km_status launch(Call<T> &c, const Resolved &r0, hipStream_t stream)
   Resolved r = r0;
   r.batch = batch;
   r.owned = (r.plan && r.plan->owned) ? 1 : 0;
+  if (r.precision != KM_PRECISION_FP64)
+  {
+    // matrix-core pass 1, FP64 pass 2 (two-pass shapes, FMA calls only)
+    KernelInfo k1, k2, kmx;
+    if (!c.square || r.exact || r.force_global ||
+        !km::twopass_kernels<T>(c.sh.d, c.sh.n[0], false, &k1, &k2) ||
+        !km::twopass_mx_kernel<T>(c.sh.d, c.sh.n[0], r.precision, &kmx))
+      return KM_ERROR_NOT_SUPPORTED;
+    if (c.slots < 1)
+      return KM_ERROR_INSUFFICIENT_WORKSPACE;
+    return launch_twopass(c, r, kmx, k2, stream);
+  }
   KernelInfo k;
   bool onchip = c.square && pick_onchip<T>(c.sh.d, c.sh.n[0], r, &k);
   if (!onchip && c.square && !r.force_global && !(r.fwd && r.exact))

@@ -127,6 +127,11 @@ template<typename T>
 bool twopass_kernels(int d, int n, bool exact, KernelInfo *sub,
                      KernelInfo *pair);
 
+// matrix-core pass 1 of the two-pass family (km_mixed.hip) for precision
+// KM_PRECISION_FP64_MFMA or KM_PRECISION_F16X4, if built for (d, n)
+template<typename T>
+bool twopass_mx_kernel(int d, int n, int precision, KernelInfo *sub);
+
 // kron_grouped for d (per-entry n up to *nmax)
 template<typename T>
 bool grouped_kernel(int d, bool exact, KernelInfo *out, int *nmax);
