@@ -206,6 +206,13 @@ def main():
                     help="stage arithmetic of the timed calls (fp64_mfma and f16x4: d6n8 only)")
     ap.add_argument("--no-precision-study", action="store_true",
                     help="skip the d=6, n=8 precision comparison added to the default line")
+    ap.add_argument("--chunks", type=int, default=8,
+                    help="xpool: entries split into this many chunks, each with its own plan, "
+                         "captured in one HIP graph and replayed (0 = skip)")
+    ap.add_argument("--xpool-variant", choices=["uniform", "x-const", "x-const-a1"], default="uniform",
+                    help="xpool counter runs: every entry reads X[0] (x-const), and also one A per "
+                         "mode (x-const-a1); the L2 requests and misses of the A pool are the "
+                         "difference of the two")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -408,6 +415,10 @@ def bench_xpool(args, dev):
     aidx = rng.integers(0, c["pool"], (batch, d))
     xidx = rng.integers(0, c["nx"], batch)
     yidx = rng.permutation(batch) // c["fan_in"]
+    if args.xpool_variant != "uniform":
+        xidx[:] = 0
+    if args.xpool_variant == "x-const-a1":
+        aidx[:] = 0
     table = km.matrix_pointer_table(pools, aidx)
     pX = km.pointer_array(X, xidx)
     pY = km.pointer_array(Y, yidx)
@@ -442,6 +453,52 @@ def bench_xpool(args, dev):
     ms_per_step, wall = results[best]
     fl = flops_per_call(d, n, batch)
     tflops = fl / (ms_per_step * 1e-3) / 1e12
+
+    # the same work in chunks, each with its own output-ordered plan,
+    # captured once in a HIP graph and replayed.  Chunks are ranges of
+    # outputs (the entries grouped by output, as an operator apply groups
+    # its elements), so every output is read and written in one chunk only.
+    graph = None
+    if args.chunks > 0:
+        k = args.chunks
+        perm = np.argsort(yidx, kind="stable")
+        ys = yidx[perm]
+        bounds = [int(np.searchsorted(ys, c["ny"] * i // k)) for i in range(k)] + [batch]
+        ctable = km.matrix_pointer_table(pools, aidx[perm])
+        cpX = km.pointer_array(X, xidx[perm])
+        cpY = km.pointer_array(Y, ys)
+        cplans = [km.Plan(d, n, 8, cpY[bounds[i]:bounds[i + 1]], bounds[i + 1] - bounds[i])
+                  for i in range(k)]
+
+        def chunked():
+            for i in range(k):
+                b0, b1 = bounds[i], bounds[i + 1]
+                km.kronmult_xbatched(d, n, ctable[d * b0:d * b1], n, cpX[b0:b1], cpY[b0:b1], None, b1 - b0,
+                                     opt=km.options(plan=cplans[i], precision=prec))
+
+        chunked()  # eager warm-up
+        torch.cuda.synchronize()
+        cg = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(cg):
+            chunked()
+        for _ in range(2):
+            cg.replay()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.steps):
+            cg.replay()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        gms = ev0.elapsed_time(ev1) / args.steps
+        graph = {"chunks": k, "ms_per_replay": round(gms, 5),
+                 "gflops": round(fl / (gms * 1e-3) / 1e9, 2),
+                 "note": "chunks = output ranges, each: plan check + kron_reg launch; all captured "
+                         "in one HIP graph"}
+        del cg
+        for cp in cplans:
+            cp.close()
     by_compulsory = 8.0 * (c["nx"] * s + c["pool"] * d * n * n + 2 * c["ny"] * s)
     by_streamed = 8.0 * (batch * s + c["pool"] * d * n * n + 2 * c["ny"] * s)
 
@@ -480,6 +537,8 @@ def bench_xpool(args, dev):
             "kernel": f"{kernel_name} ({nt} threads/workgroup)",
             "hbm_gbs_compulsory": round(by_compulsory / (ms_per_step * 1e-3) / 1e9, 1),
             "hbm_gbs_x_per_entry": round(by_streamed / (ms_per_step * 1e-3) / 1e9, 1),
+            "graph_chunks": graph,
+            "xpool_variant": args.xpool_variant,
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -493,10 +552,6 @@ def bench_xpool(args, dev):
         "cpu_baseline": cpu,
         "wall_s": round(wall, 4),
     }
-    if args.precision != "fp64":
-        out["config"]["precision"] = args.precision
-    if study is not None:
-        out["precision_study"] = study
     print(json.dumps(out))
     if plan is not None:
         plan.close()
@@ -650,10 +705,6 @@ def bench_grouped(args, dev):
         "cpu_baseline": cpu,
         "wall_s": round(wall, 4),
     }
-    if args.precision != "fp64":
-        out["config"]["precision"] = args.precision
-    if study is not None:
-        out["precision_study"] = study
     print(json.dumps(out))
     plan.close()
 
