@@ -8,6 +8,8 @@
 //   km_kronmult_vbatched_*  kronmult6_vbatched.hpp:10-34 (template only there)
 #include <algorithm>
 #include <atomic>
+#include <link.h>
+#include <sys/stat.h>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -101,6 +103,88 @@ void set_error(const char *where, hipError_t e)
       return KM_ERROR_HIP;                                                     \
     }                                                                          \
   } while (0)
+
+// ---------------------------------------------------------------------------
+// One HIP runtime per process.  A second copy of the runtime library (for
+// instance PyTorch's own, loaded under another file name after this library
+// bound the system one) keeps separate device state: pointers and streams of
+// one are not valid in the other, and calls across them crash.  Entry points
+// that touch HIP check for it first and return KM_ERROR_HIP instead.  The
+// loaded objects are listed again only when the dynamic loader's counters of
+// loads and unloads change.
+// ---------------------------------------------------------------------------
+struct RuntimeScan
+{
+  std::vector<std::pair<unsigned long, unsigned long>> files; // (dev, inode)
+  std::string names;
+};
+
+int scan_runtime(struct dl_phdr_info *info, size_t, void *p)
+{
+  RuntimeScan *r = static_cast<RuntimeScan *>(p);
+  const char *nm = info->dlpi_name;
+  if (!nm || !*nm)
+    return 0;
+  const char *base = std::strrchr(nm, '/');
+  base = base ? base + 1 : nm;
+  if (std::strncmp(base, "libamdhip64.so", 14) != 0)
+    return 0;
+  struct stat st;
+  if (stat(nm, &st) != 0)
+    return 0;
+  const std::pair<unsigned long, unsigned long> id((unsigned long)st.st_dev,
+                                                   (unsigned long)st.st_ino);
+  if (std::find(r->files.begin(), r->files.end(), id) == r->files.end())
+  {
+    r->files.push_back(id);
+    r->names += r->names.empty() ? "" : ", ";
+    r->names += nm;
+  }
+  return 0;
+}
+
+int loader_generation(struct dl_phdr_info *info, size_t size, void *p)
+{
+  unsigned long long *g = static_cast<unsigned long long *>(p);
+  if (size >= offsetof(struct dl_phdr_info, dlpi_subs) + sizeof(info->dlpi_subs))
+    *g = (unsigned long long)info->dlpi_adds * 1000003ull + info->dlpi_subs;
+  return 1; // the counters are the same in every entry: stop at the first
+}
+
+} // namespace
+
+namespace km
+{
+km_status runtime_check()
+{
+  static std::mutex mu;
+  static unsigned long long seen = ~0ull;
+  static bool two = false;
+  static std::string names;
+  unsigned long long gen = 0;
+  dl_iterate_phdr(loader_generation, &gen);
+  std::lock_guard<std::mutex> lk(mu);
+  if (gen != seen || gen == 0)
+  {
+    RuntimeScan r;
+    dl_iterate_phdr(scan_runtime, &r);
+    two = r.files.size() > 1;
+    names = r.names;
+    seen = gen;
+  }
+  if (!two)
+    return KM_SUCCESS;
+  std::snprintf(g_last_error, sizeof(g_last_error),
+                "two HIP runtimes are loaded in this process (%s): load the "
+                "framework that owns the device buffers (e.g. PyTorch) before "
+                "this library, so that both use one runtime",
+                names.c_str());
+  return KM_ERROR_HIP;
+}
+} // namespace km
+
+namespace
+{
 
 int device_cus()
 {
@@ -484,6 +568,8 @@ km_status launch(Call<T> &c, const Resolved &r0, hipStream_t stream)
   const int batch = c.a.batch;
   if (batch == 0)
     return KM_SUCCESS;
+  if (km_status g = km::runtime_check(); g != KM_SUCCESS)
+    return g;
   Resolved r = r0;
   r.batch = batch;
   r.owned = (r.plan && r.plan->owned) ? 1 : 0;
@@ -795,6 +881,8 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     return KM_SUCCESS;
   if (!Aarray || !pX || !pY)
     return KM_ERROR_INVALID_VALUE;
+  if (km_status g = km::runtime_check(); g != KM_SUCCESS)
+    return g;
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (dev != plan->device)
@@ -952,6 +1040,8 @@ km_status kgemm(int op, int mm, int nn, int kk, T alpha, const T *const *A,
     o = *opt;
   if (o.precision != KM_PRECISION_FP64 || o.plan)
     return KM_ERROR_NOT_SUPPORTED;
+  if (km_status rc = km::runtime_check(); rc != KM_SUCCESS)
+    return rc;
   const int cus = device_cus();
   if (cus <= 0)
   {
@@ -992,6 +1082,8 @@ km_status xbatched_tables(int d, int batch, const T *const *Apool,
       !Aarray || !pX || !pY || x_stride < 0 || y_stride < 0 || x_count < 0 ||
       y_count < 0)
     return KM_ERROR_INVALID_VALUE;
+  if (km_status rc = km::runtime_check(); rc != KM_SUCCESS)
+    return rc;
   km::TableArgs<T> t{};
   t.d = d;
   t.batch = batch;
@@ -1093,6 +1185,8 @@ km_status pbatched_noalloc(int d, int n, const T *A, T *const *pX,
                   batch, G);
     return KM_ERROR_INVALID_VALUE;
   }
+  if (km_status g = km::runtime_check(); g != KM_SUCCESS)
+    return g;
   void *buf = nullptr;
   KM_HIP(km::noalloc_scratch(&buf));
   km::OrderOut o{};
@@ -1246,6 +1340,8 @@ extern "C"
     *plan = nullptr;
     km_status s = check_dn(d, n, batchCount);
     if (s != KM_SUCCESS)
+      return s;
+    if ((s = km::runtime_check()) != KM_SUCCESS)
       return s;
     hipStream_t st = (hipStream_t)stream;
     const long long G = plan_groups(d, n, elem_bytes, batchCount);
@@ -1465,6 +1561,8 @@ extern "C"
         (elem_bytes != 4 && elem_bytes != 8))
       return KM_ERROR_INVALID_VALUE;
     *plan = nullptr;
+    if (km_status g = km::runtime_check(); g != KM_SUCCESS)
+      return g;
     KernelInfo k;
     int nmax = 0;
     bool ok = (elem_bytes == 8) ? km::grouped_kernel<double>(d, false, &k, &nmax)

@@ -175,6 +175,10 @@ int tuned_variant(int d, int n, int elem_bytes, int owned, long long batch);
 template<typename T>
 bool variant_kernel(int v, int d, int n, KernelInfo *k);
 
+// KM_ERROR_HIP (with km_last_error text) when two HIP runtime libraries are
+// loaded in the process, else KM_SUCCESS (km_api.cpp)
+km_status runtime_check();
+
 // device address of the allocation-free entry point's static buffer on the
 // current device (KM_NOALLOC_SCRATCH_BYTES bytes; km_noalloc.hip)
 hipError_t noalloc_scratch(void **ptr);

@@ -325,6 +325,8 @@ extern "C"
     if (d < 1 || d > 6 || n < 1 || batchCount < 1 || overlap < 1 ||
         (elem_bytes != 4 && elem_bytes != 8))
       return KM_ERROR_INVALID_VALUE;
+    if (km_status g = runtime_check(); g != KM_SUCCESS)
+      return g;
     return elem_bytes == 8
                ? tune<double>(d, n, batchCount, overlap, owned, (hipStream_t)stream, variant)
                : tune<float>(d, n, batchCount, overlap, owned, (hipStream_t)stream, variant);

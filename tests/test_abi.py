@@ -8,6 +8,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -191,3 +192,36 @@ def test_tune_table_json_round_trip(tmp_path):
         assert len(doc["entries"]) == 2
     finally:
         km.tune_clear()
+
+
+_TWO_RUNTIMES = r'''
+import ctypes, os, sys
+lib = ctypes.CDLL(sys.argv[1])          # binds the system HIP runtime first
+if sys.argv[2] == "second":
+    ctypes.CDLL(sys.argv[3])            # a second copy of the runtime library
+f = lib.km_kronmult_pbatched_f64
+f.argtypes = [ctypes.c_int] * 2 + [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+lib.km_last_error.restype = ctypes.c_char_p
+st = f(3, 4, 8, 8, 8, None, 2, None, None)
+print(st, lib.km_last_error().decode())
+'''
+
+
+def test_second_hip_runtime_is_reported(tmp_path):
+    """A process holding two HIP runtime libraries (this library loaded
+    before PyTorch's own copy) gets KM_ERROR_HIP with an explanation instead
+    of mixing device state across runtimes; one runtime is not reported."""
+    import torch
+    second = os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so")
+    if not os.path.exists(second):
+        pytest.skip("no second HIP runtime library to load")
+    script = tmp_path / "two.py"
+    script.write_text(_TWO_RUNTIMES)
+    outs = {}
+    for mode in ("one", "second"):
+        r = subprocess.run([sys.executable, str(script), LIB, mode, second], capture_output=True, text=True,
+                           timeout=120, env={**os.environ, "HIP_VISIBLE_DEVICES": os.environ.get("HIP_VISIBLE_DEVICES", "")})
+        assert r.returncode == 0, r.stderr
+        outs[mode] = r.stdout.strip().splitlines()[-1]
+    assert outs["second"].startswith("4 ") and "two HIP runtimes" in outs["second"], outs
+    assert "two HIP runtimes" not in outs["one"], outs

@@ -1,7 +1,9 @@
 """GPU run of the C++ caller of the reference-named interface
-(tests/compat/caller.cpp, built by __graft_entry__.build()): kronmult3_batched,
-kronmult3_pbatched with shared outputs and kronmult3_xbatched with a pooled,
-padded A, each against an explicit Kronecker matrix on the host."""
+(tests/compat/caller.cpp, built by __graft_entry__.build()): every overload,
+kronmult{1..6}_{batched,pbatched,xbatched,vbatched} (shared outputs, a pooled
+and padded A, rectangular vbatched factors) and kgemm_{nn,nt,tt}_batched
+(beta 0, 1 with a shared C, 0.5), each against an explicit Kronecker matrix or
+product on the host."""
 from __future__ import annotations
 
 import os
@@ -21,4 +23,7 @@ def test_cpp_caller_runs():
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.strip().splitlines()
     assert lines[-1] == "ok", r.stdout
-    assert sum(line.startswith("kronmult3_") for line in lines) == 3
+    for d in range(1, 7):
+        for what in ("batched", "pbatched", "xbatched", "vbatched"):
+            assert sum(line.startswith(f"kronmult{d}_{what} ") for line in lines) == 1, (d, what, r.stdout)
+    assert sum(line.startswith("kgemm_") for line in lines) == 9, r.stdout
