@@ -12,9 +12,9 @@
 //   kronmultN_vbatched  kronmult6_vbatched.hpp:10-34  (a __global__ template in
 //                       the reference; here a host function with its arguments)
 // Size limit: shapes served by the workspace-staged kernel (rectangular
-// vbatched factors, and square n^d too large for the on-chip kernels with
-// n > 8) take n_k <= 32; a larger n_k is reported as "not supported" and the
-// call aborts.  The reference has no such limit.
+// vbatched factors, and square shapes without an on-chip kernel) take
+// n_k <= 300; a larger n_k is reported as "not supported" and the call
+// aborts.  The reference has no such limit.
 #ifndef KRONMULT_COMPAT_HPP
 #define KRONMULT_COMPAT_HPP
 

@@ -75,6 +75,9 @@ struct km_group_plan_s
   std::vector<hipStream_t> streams;
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins;
+  // held across a call's fork, launches and join: calls from several host
+  // threads on one plan take turns with its streams and events
+  std::mutex fork_mu;
 };
 
 namespace
@@ -607,14 +610,27 @@ km_status launch(Call<T> &c, const Resolved &r0, hipStream_t stream)
       return launch_twopass_fwd(c, r, k1, k2, stream);
     }
   }
+  // the workspace-staged kernel caches fibers wider than KM_GLOBAL_MAX_N in
+  // dynamic LDS, one slice of nmax values per thread: fewer threads per
+  // workgroup for wider modes
+  size_t global_lds = 0;
   if (!onchip)
   {
-    for (int i = 0; i < c.sh.d; i++)
-      if (c.sh.n[i] > KM_GLOBAL_MAX_N)
-        return KM_ERROR_NOT_SUPPORTED;
     if (c.slots < 1)
       return KM_ERROR_INSUFFICIENT_WORKSPACE;
     k = km::global_kernel<T>(r.exact);
+    int nmax = 0;
+    for (int i = 0; i < c.sh.d; i++)
+      nmax = std::max(nmax, c.sh.n[i]);
+    if (nmax > KM_GLOBAL_MAX_N)
+    {
+      const size_t per = sizeof(typename km::Acc<T>::type) * (size_t)nmax;
+      const int nt = (int)std::min<size_t>(256, KM_GLOBAL_LDS_BYTES / per / 64 * 64);
+      if (nt < 64)
+        return KM_ERROR_NOT_SUPPORTED; // n_k above 300
+      k.nt = nt;
+      global_lds = per * nt;
+    }
   }
   int cus = device_cus();
   if (cus <= 0)
@@ -678,7 +694,7 @@ km_status launch(Call<T> &c, const Resolved &r0, hipStream_t stream)
   {
     int fwd = r.fwd ? 1 : 0;
     void *args[] = {&c.a, &c.sh, &fwd};
-    e = hipLaunchKernel(k.fn, dim3(grid), dim3(k.nt), args, 0, stream);
+    e = hipLaunchKernel(k.fn, dim3(grid), dim3(k.nt), args, global_lds, stream);
   }
   if (e != hipSuccess)
   {
@@ -948,8 +964,10 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     // The even split of the atomic path already sizes each bin's grid to
     // fill the chip, and there the fork and join only add latency.
     const int ns = owned ? (int)plan->streams.size() : 0;
+    std::unique_lock<std::mutex> fork_lock(plan->fork_mu, std::defer_lock);
     if (ns > 1)
     {
+      fork_lock.lock();
       KM_HIP(hipEventRecord(plan->fork, (hipStream_t)stream));
       for (hipStream_t st : plan->streams)
         KM_HIP(hipStreamWaitEvent(st, plan->fork, 0));

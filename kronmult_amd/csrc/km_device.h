@@ -35,8 +35,11 @@
 namespace km
 {
 
-// largest mode extent n_k accepted by the workspace-staged kernel
+// largest mode extent n_k whose fiber the workspace-staged kernel holds in
+// registers; wider fibers go to dynamic LDS (KM_GLOBAL_LDS_BYTES per
+// workgroup, which bounds n_k * workgroup size)
 #define KM_GLOBAL_MAX_N 32
+#define KM_GLOBAL_LDS_BYTES (150 * 1024)
 
 // fiber-loop unroll factors of the LDS stage passes (tuning knobs)
 #ifndef KM_LDS_UNROLL
@@ -1913,8 +1916,8 @@ __global__ __launch_bounds__(NT) void kron_sub_fwd(EntryArgs<T> a, int e0, int e
 }
 
 // ---------------------------------------------------------------------------
-// kron_global: any d <= 6 and any (rectangular) m_k x n_k with
-// n_k <= KM_GLOBAL_MAX_N; one workgroup per entry; stages go through the caller's workspace as in the reference
+// kron_global: any d <= 6 and any (rectangular) m_k x n_k; one workgroup per
+// entry; stages go through the caller's workspace as in the reference
 // (kronm_backward.hpp:92-139).  Square shapes update one buffer in place;
 // rectangular shapes ping-pong between the slot's two buffers.  The final
 // stage adds into Y with atomics, or with a plain read-modify-write when the
@@ -1927,8 +1930,12 @@ __global__ __launch_bounds__(256) void kron_global(EntryArgs<T> a, Shape sh,
 {
   plan_bind(a);
   using C = typename Acc<T>::type;
+  extern __shared__ __attribute__((aligned(16))) unsigned char kg_dyn[];
   const int tid = threadIdx.x;
   const int d = sh.d;
+  int nmax = 0;
+  for (int k = 0; k < d; k++)
+    nmax = max(nmax, sh.n[k]);
   int p0, p1;
   wg_range(a, p0, p1);
   bool square = true;
@@ -1970,19 +1977,43 @@ __global__ __launch_bounds__(256) void kron_global(EntryArgs<T> a, Shape sh,
       {
         long long hi = f / inner, lo = f - hi * inner;
         const T *xin = src + hi * inner * nk + lo;
-        C x[KM_GLOBAL_MAX_N];
-        for (int j = 0; j < nk; j++)
-          x[j] = static_cast<C>(gld(xin + (long long)j * inner));
-        for (int i = 0; i < mk; i++)
+        if (nk <= KM_GLOBAL_MAX_N)
         {
-          C acc = 0;
+          // the fiber in registers
+          C x[KM_GLOBAL_MAX_N];
           for (int j = 0; j < nk; j++)
-            acc = madd<EXACT>(acc, static_cast<C>(cld(Ak + (i + j * ld))), x[j]);
-          long long o = hi * inner * mk + lo + (long long)i * inner;
-          if (last)
-            flush_add(yb + o, static_cast<T>(acc), a.owned);
-          else
-            gst(dst + o, static_cast<T>(acc));
+            x[j] = static_cast<C>(gld(xin + (long long)j * inner));
+          for (int i = 0; i < mk; i++)
+          {
+            C acc = 0;
+            for (int j = 0; j < nk; j++)
+              acc = madd<EXACT>(acc, static_cast<C>(cld(Ak + (i + j * ld))), x[j]);
+            long long o = hi * inner * mk + lo + (long long)i * inner;
+            if (last)
+              flush_add(yb + o, static_cast<T>(acc), a.owned);
+            else
+              gst(dst + o, static_cast<T>(acc));
+          }
+        }
+        else
+        {
+          // wider modes: the fiber goes to this thread's slice of the
+          // dynamic LDS buffer (blockDim.x * nmax values), so square shapes
+          // may still update their buffer in place
+          C *xf = reinterpret_cast<C *>(kg_dyn) + (size_t)tid * nmax;
+          for (int j = 0; j < nk; j++)
+            xf[j] = static_cast<C>(gld(xin + (long long)j * inner));
+          for (int i = 0; i < mk; i++)
+          {
+            C acc = 0;
+            for (int j = 0; j < nk; j++)
+              acc = madd<EXACT>(acc, static_cast<C>(cld(Ak + (i + j * ld))), xf[j]);
+            long long o = hi * inner * mk + lo + (long long)i * inner;
+            if (last)
+              flush_add(yb + o, static_cast<T>(acc), a.owned);
+            else
+              gst(dst + o, static_cast<T>(acc));
+          }
         }
       }
       __syncthreads();
