@@ -1062,15 +1062,35 @@ __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
     }
     return p0 + q;
   };
+  // Owned runs all switch at the same slab, and a switch (the old run's
+  // stores plus the new run's Y loads, 2*N*SL elements) takes one CU about
+  // 13 us beyond a plain slab even alone; with every CU at once it took
+  // 23 us.  At d6n6 each workgroup therefore starts its slab sequence at
+  // slab (5*blockIdx) mod nslab and wraps around: switches fall at
+  // different times, and the run it starts inside is visited twice (its
+  // second visit preloads what the first stored).  Measured: d6n6 plan
+  // 0.770 -> 0.746 ms.  d5n8 has two runs per workgroup, so the extra visit
+  // costs more than the spread saves (0.189 -> 0.194 ms): off there.
+  constexpr int kStagger = (PRELOAD && D == 6) ? 5 : 0;
+  const int srot =
+      kStagger ? (int)((blockIdx.x * (unsigned)kStagger) % (unsigned)nslab) : 0;
+  auto slab_of = [&](int t, int &j) {
+    int u = t + srot;
+    u = u >= nslab ? u - nslab : u;
+    j = u - (u / N) * N;
+    return pos(u / N);
+  };
   {
-    const T *xp = x_of(a, entry_at(a, pos(0)));
+    int j0;
+    const int p0s = slab_of(0, j0);
+    const T *xp = x_of(a, entry_at(a, p0s)) + (size_t)j0 * SL;
     slab_async_load<T, SL, NT, XNT>(sb0, xp, tid);
   }
 
   for (int t = 0; t < nslab; ++t)
   {
-    const int p = pos(t / N);
-    const int j1 = t - (t / N) * N;
+    int j1;
+    const int p = slab_of(t, j1);
     const int b = entry_at(a, p);
     T *cur = (t & 1) ? sb1 : sb0;
     T *nxt = (t & 1) ? sb0 : sb1;
@@ -1080,7 +1100,7 @@ __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
     // issued a whole slab earlier (below), so they drain behind compute.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (j1 == 0)
+    if (j1 == 0 || t == 0)
     {
       T *yb = y_of(a, b);
       if (yb != target)
@@ -1100,8 +1120,8 @@ __device__ __forceinline__ void slab_body(const EntryArgs<T> &a, T *sb0, T *sb1,
     }
     if (t + 1 < nslab)
     {
-      const int pn = pos((t + 1) / N);
-      const int jn = (t + 1) - ((t + 1) / N) * N;
+      int jn;
+      const int pn = slab_of(t + 1, jn);
       const T *xp = x_of(a, entry_at(a, pn));
       slab_async_load<T, SL, NT, XNT>(nxt, xp + (size_t)jn * SL, tid);
     }
