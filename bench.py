@@ -247,7 +247,7 @@ def main():
     pY = km.pointer_array(Y, yidx)
     fam, nt = km.query_kernel(d, n, 8, None)
     # the workspace kernels (two-pass, generic) use one n^d slot per entry
-    W = torch.empty((batch, s), device=dev, dtype=torch.float64) if fam in (2, 3) else None
+    W = torch.empty((batch, s), device=dev, dtype=torch.float64) if fam in (2, 3, 7) else None
     pW = km.pointer_array(W, np.arange(batch)) if W is not None else None
 
     # a plan is set-up work (once per output pointer array), outside the
@@ -259,7 +259,7 @@ def main():
     todo = ["atomic", "plan"] if args.mode == "best" else [args.mode]
 
     kernel_name = {0: "kron_lds", 1: "kron_slab", 2: "kron_global", 3: "kron_sub+kron_pair",
-                   4: "kron_reg"}[fam]
+                   4: "kron_reg", 6: "kron_half", 7: "kron_xcd"}[fam]
 
     stream = torch.cuda.current_stream()
 

@@ -322,7 +322,13 @@ typedef enum km_kernel_variant
   KM_KERNEL_HALF2 = 3,    /* the same, two slabs per step                    */
   KM_KERNEL_REG = 4,      /* register-blocked mode groups                    */
   KM_KERNEL_LDS = 5,      /* whole tensor in LDS                             */
-  KM_KERNEL_LDS_WIDE = 6  /* whole tensor in LDS, wider workgroups           */
+  KM_KERNEL_LDS_WIDE = 6, /* whole tensor in LDS, wider workgroups           */
+  KM_KERNEL_TWOPASS = 7,  /* n^(d-1) above LDS (d = 6, n = 7, 8): two
+                             kernels, W through HBM                         */
+  KM_KERNEL_XCD = 8       /* the same in one cooperative launch, W in each
+                             XCD's L2 (gfx950); half the HBM bytes, but
+                             slower where measured, so never the built-in
+                             choice                                         */
 } km_kernel_variant;
 
 km_status km_tune_shape(int d, int n, int elem_bytes, int batchCount,
@@ -340,7 +346,11 @@ km_status km_tune_load(const char *path);
  *   *family: 0 = LDS-resident, 1 = slab-streamed, 2 = workspace-staged
  *            (generic), 3 = two-pass (sub-slabs in LDS, then the two
  *            leading modes in registers; uses W / pW), 4 = register-blocked
- *            (mode groups in registers, LDS transposes between them).
+ *            (mode groups in registers, LDS transposes between them),
+ *            7 = the two-pass family fused into one cooperative launch on
+ *            gfx950 (each XCD runs its share of the entries with W in its
+ *            L2; FMA calls whose entries each bring a workspace), used where
+ *            the variant table selects KM_KERNEL_XCD.
  *   *threads: workgroup size.
  * ------------------------------------------------------------------------ */
 km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
@@ -351,6 +361,15 @@ km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
 km_status km_query_kernel_for(int d, int n, int elem_bytes, int batchCount,
                               int owned, const km_options *opt, int *family,
                               int *threads);
+
+/* Launches of the fused two-pass kernel (family 7) on the current device
+ * whose bounded inter-workgroup waits gave up, since the library was loaded
+ * (synchronous).  Such a launch exits without completing its outputs; it
+ * cannot happen while every workgroup of the cooperative launch is resident,
+ * which the launch guarantees.  No counterpart in the reference (whose
+ * two-pass chain is separate kernels, kronm_backward.hpp:92-139); a check
+ * for callers and tests of the fused path. */
+km_status km_cooperative_faults(unsigned long long *count);
 
 /* Reference flop model (kron_flops, kroncommon.hpp:154-185) for one entry. */
 double km_kron_flops(int d, const int *m, const int *n);

@@ -52,7 +52,7 @@ FLAG_PLAN_OUTPUTS = 0x10
 
 # kernel families reported by query_kernel
 FAMILY_LDS, FAMILY_SLAB, FAMILY_GLOBAL, FAMILY_TWOPASS, FAMILY_REG = 0, 1, 2, 3, 4
-FAMILY_GROUPED, FAMILY_HALF = 5, 6
+FAMILY_GROUPED, FAMILY_HALF, FAMILY_XCD = 5, 6, 7
 
 # every symbol the C ABI exports (include/kronmult_amd.h)
 C_ABI_SYMBOLS = (
@@ -70,11 +70,12 @@ C_ABI_SYMBOLS = (
     "km_xbatched_tables_f64", "km_xbatched_tables_f32",
     "km_kronmult_pbatched_noalloc_f64",
     "km_tune_shape", "km_tune_set", "km_tune_get", "km_tune_clear", "km_tune_save", "km_tune_load",
-    "km_query_kernel_for",
+    "km_query_kernel_for", "km_cooperative_faults",
 )
 
 # km_kernel_variant
-KERNEL_DEFAULT, KERNEL_SLAB, KERNEL_HALF1, KERNEL_HALF2, KERNEL_REG, KERNEL_LDS, KERNEL_LDS_WIDE = range(7)
+(KERNEL_DEFAULT, KERNEL_SLAB, KERNEL_HALF1, KERNEL_HALF2, KERNEL_REG, KERNEL_LDS, KERNEL_LDS_WIDE,
+ KERNEL_TWOPASS, KERNEL_XCD) = range(9)
 
 # largest batchCount of kronmult_pbatched_noalloc (KM_NOALLOC_MAX_BATCH)
 NOALLOC_MAX_BATCH = 262144
@@ -162,6 +163,8 @@ def lib():
         L.km_query_kernel.restype = _I
         L.km_query_kernel_for.argtypes = [_I, _I, _I, _I, _I, C.POINTER(Options), C.POINTER(_I), C.POINTER(_I)]
         L.km_query_kernel_for.restype = _I
+        L.km_cooperative_faults.argtypes = [C.POINTER(C.c_ulonglong)]
+        L.km_cooperative_faults.restype = _I
         L.km_tune_shape.argtypes = [_I, _I, _I, _I, _I, _I, _P, C.POINTER(_I)]
         L.km_tune_set.argtypes = [_I, _I, _I, _I, _I, _I]
         L.km_tune_get.argtypes = [_I, _I, _I, _I, _I, C.POINTER(_I)]
@@ -448,6 +451,14 @@ def query_kernel_for(d: int, n: int, batchCount: int, owned: bool = False, elem_
     _check(lib().km_query_kernel_for(d, n, elem_bytes, batchCount, int(owned), _opt_ref(opt),
                                      C.byref(fam), C.byref(nt)), "km_query_kernel_for")
     return fam.value, nt.value
+
+
+def cooperative_faults() -> int:
+    """Launches of the fused two-pass kernel on the current device whose
+    bounded inter-workgroup waits gave up (km_cooperative_faults)."""
+    v = C.c_ulonglong()
+    _check(lib().km_cooperative_faults(C.byref(v)), "km_cooperative_faults")
+    return v.value
 
 
 def tune_shape(d: int, n: int, batchCount: int, overlap: int = 1, owned: bool = False,

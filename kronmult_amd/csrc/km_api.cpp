@@ -228,6 +228,32 @@ int occupancy(const void *fn, int nt)
   return nb;
 }
 
+// kron_xcd runs on MI355X's layout (gfx950: eight XCDs, the CUs split evenly
+// over them) and needs cooperative launches
+bool xcd_device_ok()
+{
+  static std::map<int, bool> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return false;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = cache.find(dev);
+    if (it != cache.end())
+      return it->second;
+  }
+  hipDeviceProp_t prop;
+  int coop = 0;
+  const bool ok =
+      hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 &&
+      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
+      coop != 0 && prop.multiProcessorCount >= 8 && prop.multiProcessorCount % 8 == 0;
+  std::lock_guard<std::mutex> lk(g_mu);
+  cache[dev] = ok;
+  return ok;
+}
+
 long long ipow(int b, int e)
 {
   long long p = 1;
@@ -335,7 +361,8 @@ bool pick_onchip(int d, int n, const Resolved &r, KernelInfo *k)
   if (!r.exact && r.batch > 0)
   {
     const int v = km::tuned_variant(d, n, (int)sizeof(T), r.owned, r.batch);
-    if (v != KM_KERNEL_DEFAULT && km::variant_kernel<T>(v, d, n, k))
+    if (v != KM_KERNEL_DEFAULT && v != KM_KERNEL_TWOPASS && v != KM_KERNEL_XCD &&
+        km::variant_kernel<T>(v, d, n, k))
       return true;
   }
   // without EXACT the contraction order only changes rounding, so the
@@ -494,6 +521,64 @@ km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
   return KM_SUCCESS;
 }
 
+// kron_xcd: the two-pass family in one cooperative launch, each XCD's W in
+// its L2 (FMA calls whose entries each bring a workspace: batched,
+// pbatched), where the variant table selects it (KM_KERNEL_XCD).  It moves
+// half the HBM bytes of the two kernels but measured slower at every size
+// tried (d6n8 x1024 2:1: 3.05 against 1.88 ms; d6n7 x16: 0.090 against
+// 0.037 ms), so the two kernels stay the built-in choice.
+// KM_ERROR_NOT_SUPPORTED leaves the call to the two kernels.  The spin limit
+// bounds every barrier wait of the kernel (about a second).
+constexpr unsigned kXcdSpinLimit = 1u << 20;
+
+template<typename T>
+km_status launch_xcd(Call<T> &c, const Resolved &r, hipStream_t stream)
+{
+  const int batch = c.a.batch;
+  const int d = c.sh.d, n = c.sh.n[0];
+  KernelInfo k;
+  if (r.exact || !c.square || !c.a.slot_by_entry || c.slots < batch ||
+      !km::xcd_kernel<T>(d, n, &k) || !xcd_device_ok())
+    return KM_ERROR_NOT_SUPPORTED;
+  if (km::tuned_variant(d, n, (int)sizeof(T), r.owned, batch) != KM_KERNEL_XCD)
+    return KM_ERROR_NOT_SUPPORTED;
+  // cooperative launches stay out of captured graphs
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return KM_ERROR_NOT_SUPPORTED;
+  const int cus = device_cus();
+  if (cus < 8)
+    return KM_ERROR_NOT_SUPPORTED;
+  if (r.plan)
+  {
+    const km_plan p = r.plan;
+    if (p->batch != batch || p->elem_bytes != (int)sizeof(T) || p->d != d ||
+        p->n != n)
+      return KM_ERROR_PLAN_MISMATCH;
+    c.a.order = p->order;
+    c.a.owned = p->owned;
+    km_status bs = bind_plan(c, r, stream);
+    if (bs != KM_SUCCESS)
+      return bs;
+  }
+  else
+    c.a.owned = 0;
+  c.a.ranges = nullptr;
+  unsigned *ctl = nullptr;
+  KM_HIP(km::xcd_prepare(stream, &ctl));
+  unsigned limit = kXcdSpinLimit;
+  void *args[] = {&c.a, &ctl, &limit};
+  hipError_t e =
+      hipLaunchCooperativeKernel(k.fn, dim3(cus * k.per_cu), dim3(k.nt), args, 0, stream);
+  if (e == hipErrorCooperativeLaunchTooLarge)
+  {
+    (void)hipGetLastError();
+    return KM_ERROR_NOT_SUPPORTED;
+  }
+  KM_HIP(e);
+  return KM_SUCCESS;
+}
+
 // Forward two-pass family (exact, mode order 1..d): pass 1 over the chunk's
 // entries and the positions of the trailing modes, pass 2 over the chunk's
 // entries and the n*n sub-slabs.
@@ -597,6 +682,9 @@ km_status launch(Call<T> &c, const Resolved &r0, hipStream_t stream)
     {
       if (c.slots < 1)
         return KM_ERROR_INSUFFICIENT_WORKSPACE;
+      const km_status xs = launch_xcd(c, r, stream);
+      if (xs != KM_ERROR_NOT_SUPPORTED)
+        return xs;
       return launch_twopass(c, r, k1, k2, stream);
     }
   }
@@ -1529,6 +1617,16 @@ extern "C"
     return KM_SUCCESS;
   }
 
+  km_status km_cooperative_faults(unsigned long long *count)
+  {
+    if (!count)
+      return KM_ERROR_INVALID_VALUE;
+    if (km_status g = km::runtime_check(); g != KM_SUCCESS)
+      return g;
+    KM_HIP(km::xcd_faults(count));
+    return KM_SUCCESS;
+  }
+
   km_status km_query_kernel(int d, int n, int elem_bytes, const km_options *opt,
                             int *family, int *threads)
   {
@@ -1561,6 +1659,14 @@ extern "C"
     else if (!onchip && !r.force_global)
       twopass = (elem_bytes == 8) ? km::twopass_fwd_kernels<double>(d, n, &k, &k2)
                                   : km::twopass_fwd_kernels<float>(d, n, &k, &k2);
+    if (twopass && !r.exact && xcd_device_ok() &&
+        km::tuned_variant(d, n, elem_bytes, r.owned, batchCount) == KM_KERNEL_XCD)
+    {
+      KernelInfo kx;
+      if ((elem_bytes == 8) ? km::xcd_kernel<double>(d, n, &kx)
+                            : km::xcd_kernel<float>(d, n, &kx))
+        k = kx;
+    }
     if (!onchip && !twopass)
       k = (elem_bytes == 8) ? km::global_kernel<double>(r.exact)
                             : km::global_kernel<float>(r.exact);

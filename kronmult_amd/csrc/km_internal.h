@@ -25,7 +25,8 @@ enum Family
   FAM_TWOPASS = 3,
   FAM_REG = 4,
   FAM_GROUPED = 5,
-  FAM_HALF = 6
+  FAM_HALF = 6,
+  FAM_XCD = 7
 };
 
 struct KernelInfo
@@ -33,6 +34,7 @@ struct KernelInfo
   const void *fn = nullptr; // __global__ function
   int nt = 0;               // threads per workgroup
   int family = FAM_LDS;
+  int per_cu = 1;           // workgroups per CU of a cooperative launch
 };
 
 // threads per workgroup of kron_lds for a tensor of s elements
@@ -126,6 +128,17 @@ bool twopass_fwd_kernels(int d, int n, KernelInfo *pair, KernelInfo *sub);
 template<typename T>
 bool twopass_kernels(int d, int n, bool exact, KernelInfo *sub,
                      KernelInfo *pair);
+
+// kron_xcd (km_xcd.hip): the two-pass family fused into one cooperative
+// launch, W in each XCD's L2 (FMA calls), if built for (d, n)
+template<typename T>
+bool xcd_kernel(int d, int n, KernelInfo *k);
+// pick the next control block of the current device and queue its reset on
+// st; the kernel takes *ctl
+hipError_t xcd_prepare(hipStream_t st, unsigned **ctl);
+// sticky count of kron_xcd launches whose bounded waits gave up (current
+// device; synchronous)
+hipError_t xcd_faults(unsigned long long *count);
 
 // matrix-core pass 1 of the two-pass family (km_mixed.hip) for precision
 // KM_PRECISION_FP64_MFMA or KM_PRECISION_F16X4, if built for (d, n)

@@ -49,7 +49,8 @@ bool g_env_loaded = false;
 // variant forced on this thread while the tuner measures a candidate
 thread_local int g_force = KM_KERNEL_DEFAULT;
 
-const char *kNames[] = {"default", "slab", "half1", "half2", "reg", "lds", "lds_wide"};
+const char *kNames[] = {"default", "slab",     "half1", "half2", "reg",
+                        "lds",     "lds_wide", "twopass", "xcd"};
 constexpr int kNumVariants = sizeof(kNames) / sizeof(kNames[0]);
 
 int variant_of(const char *s)
@@ -155,6 +156,11 @@ std::vector<int> candidates(int d, int n)
 {
   std::vector<int> c;
   KernelInfo k;
+  if (xcd_kernel<T>(d, n, &k))
+  {
+    c = {KM_KERNEL_XCD, KM_KERNEL_TWOPASS};
+    return c;
+  }
   if (half_kernel<T>(d, n, 1, &k))
   {
     c = {KM_KERNEL_SLAB, KM_KERNEL_HALF1, KM_KERNEL_HALF2};
@@ -180,6 +186,12 @@ bool variant_kernel(int v, int d, int n, KernelInfo *k)
   case KM_KERNEL_REG: return reg_kernel<T>(d, n, false, k);
   case KM_KERNEL_LDS: return lds_kernel<T>(d, n, false, false, k);
   case KM_KERNEL_LDS_WIDE: return lds_kernel_wide<T>(d, n, k);
+  case KM_KERNEL_TWOPASS:
+  {
+    KernelInfo k2;
+    return twopass_kernels<T>(d, n, false, k, &k2);
+  }
+  case KM_KERNEL_XCD: return xcd_kernel<T>(d, n, k);
   default: return false;
   }
 }
@@ -210,8 +222,11 @@ km_status measure(int d, int n, int batch, int overlap, int owned, int v,
   for (int i = 0; i < d; i++)
     s *= n;
   const int nY = std::max(1, batch / std::max(1, overlap));
-  T *A = nullptr, *X = nullptr, *Y = nullptr;
-  T **pX = nullptr, **pY = nullptr;
+  T *A = nullptr, *X = nullptr, *Y = nullptr, *W = nullptr;
+  T **pX = nullptr, **pY = nullptr, **pW = nullptr;
+  // the two-pass shapes take one n^d workspace per entry
+  KernelInfo kw;
+  const bool need_w = xcd_kernel<T>(d, n, &kw);
   km_plan plan = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   km_status st2 = KM_SUCCESS;
@@ -224,16 +239,20 @@ km_status measure(int d, int n, int batch, int overlap, int owned, int v,
   const size_t xbytes = sizeof(T) * (size_t)batch * s;
   const size_t ybytes = sizeof(T) * (size_t)nY * s;
   if (hip(hipMalloc(&A, abytes)) && hip(hipMalloc(&X, xbytes)) && hip(hipMalloc(&Y, ybytes)) &&
-      hip(hipMalloc(&pX, sizeof(T *) * batch)) && hip(hipMalloc(&pY, sizeof(T *) * batch)))
+      hip(hipMalloc(&pX, sizeof(T *) * batch)) && hip(hipMalloc(&pY, sizeof(T *) * batch)) &&
+      (!need_w || (hip(hipMalloc(&W, xbytes)) && hip(hipMalloc(&pW, sizeof(T *) * batch)))))
   {
-    std::vector<T *> hx(batch), hy(batch);
+    std::vector<T *> hx(batch), hy(batch), hw(batch);
     for (int b = 0; b < batch; b++)
     {
       hx[b] = X + (size_t)b * s;
       hy[b] = Y + (size_t)(b / std::max(1, overlap) % nY) * s;
+      hw[b] = need_w ? W + (size_t)b * s : nullptr;
     }
     hip(hipMemcpy(pX, hx.data(), sizeof(T *) * batch, hipMemcpyHostToDevice));
     hip(hipMemcpy(pY, hy.data(), sizeof(T *) * batch, hipMemcpyHostToDevice));
+    if (need_w)
+      hip(hipMemcpy(pW, hw.data(), sizeof(T *) * batch, hipMemcpyHostToDevice));
     if (st2 == KM_SUCCESS)
       st2 = fill(A, abytes, st);
     if (st2 == KM_SUCCESS)
@@ -249,9 +268,11 @@ km_status measure(int d, int n, int batch, int overlap, int owned, int v,
     auto call = [&]() {
       return (sizeof(T) == 8)
                  ? km_kronmult_pbatched_f64(d, n, (const double *)A, (double *const *)pX,
-                                            (double *const *)pY, nullptr, batch, st, &o)
+                                            (double *const *)pY, (double *const *)pW, batch,
+                                            st, &o)
                  : km_kronmult_pbatched_f32(d, n, (const float *)A, (float *const *)pX,
-                                            (float *const *)pY, nullptr, batch, st, &o);
+                                            (float *const *)pY, (float *const *)pW, batch, st,
+                                            &o);
     };
     for (int w = 0; w < 3 && st2 == KM_SUCCESS; w++)
       st2 = call();
@@ -280,6 +301,8 @@ km_status measure(int d, int n, int batch, int overlap, int owned, int v,
   (void)hipFree(Y);
   (void)hipFree(pX);
   (void)hipFree(pY);
+  (void)hipFree(W);
+  (void)hipFree(pW);
   return st2;
 }
 

@@ -61,6 +61,19 @@ def test_tune_shape_save_load_honoured(km, tmp_path):
     km.tune_clear()
 
 
+def test_tune_two_pass_shape(km):
+    """d6n7 (one slab above LDS): the autotuner times the two-kernel form and
+    the fused cooperative kernel, each call with a workspace per entry, and
+    records the faster; the call then reports that family."""
+    km.tune_clear()
+    v = km.tune_shape(6, 7, 32, overlap=2)
+    assert v in (km.KERNEL_TWOPASS, km.KERNEL_XCD)
+    fam = km.FAMILY_TWOPASS if v == km.KERNEL_TWOPASS else km.FAMILY_XCD
+    assert km.query_kernel_for(6, 7, 32)[0] == fam
+    km.tune_clear()
+    assert km.query_kernel_for(6, 7, 32)[0] == km.FAMILY_TWOPASS
+
+
 VARIANTS = [(6, 6, "HALF1"), (6, 6, "HALF2"), (5, 8, "HALF1"), (5, 8, "HALF2"), (6, 6, "SLAB"),
             (6, 4, "LDS"), (4, 5, "LDS_WIDE"), (3, 8, "LDS"), (4, 4, "REG")]
 
