@@ -62,7 +62,10 @@ struct km_group_plan_s
   int noutputs = 0;
   int *nvec = nullptr;   // device, per entry
   int *order = nullptr;  // device, batch entries
-  int *ranges = nullptr; // device, ngroups + 1 entries (persistent kernel)
+  // one-launch kernel (kron_runs): workgroup w takes output runs
+  // [ranges[w], ranges[w+1]); run r starts at order position runstart[r]
+  int *ranges = nullptr;   // device, ngroups + 1 entries
+  int *runstart = nullptr; // device, noutputs + 1 entries
   // per-size-bin launch: bin b uses binranges[first .. first+count]
   std::vector<km_group_bin> bins;
   int *binranges = nullptr; // device
@@ -1107,8 +1110,8 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     return KM_SUCCESS;
   }
   KernelInfo k;
-  int nmax = 0;
-  if (!km::grouped_kernel<T>(plan->d, exact, &k, &nmax) || plan->nmax > nmax)
+  int nmax = 0, small_max = 0;
+  if (!km::runs_kernel<T>(plan->d, exact, &k, &nmax, &small_max) || plan->nmax > nmax)
     return KM_ERROR_NOT_SUPPORTED;
   EntryArgs<T> a{};
   a.Aptr = Aarray;
@@ -1122,7 +1125,8 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
   a.bind_flag = bind_flag;
   a.bind_gen = bind_gen;
   const int *nv = plan->nvec;
-  void *args[] = {&a, &nv};
+  const int *rs = plan->runstart;
+  void *args[] = {&a, &nv, &rs};
   KM_HIP(hipLaunchKernel(k.fn, dim3(plan->ngroups), dim3(k.nt), args, 0,
                          (hipStream_t)stream));
   return KM_SUCCESS;
@@ -1688,9 +1692,9 @@ extern "C"
     if (km_status g = km::runtime_check(); g != KM_SUCCESS)
       return g;
     KernelInfo k;
-    int nmax = 0;
-    bool ok = (elem_bytes == 8) ? km::grouped_kernel<double>(d, false, &k, &nmax)
-                                : km::grouped_kernel<float>(d, false, &k, &nmax);
+    int nmax = 0, small_max = 0;
+    bool ok = (elem_bytes == 8) ? km::runs_kernel<double>(d, false, &k, &nmax, &small_max)
+                                : km::runs_kernel<float>(d, false, &k, &nmax, &small_max);
     if (!ok)
       return KM_ERROR_NOT_SUPPORTED;
     int pn = 0;
@@ -1737,15 +1741,17 @@ extern "C"
         return n_host[x] > n_host[y];
       return ys[x] < ys[y];
     });
-    // cost of an entry ~ its flops 2 d n^(d+1) plus a fixed per-entry term;
-    // entries small enough to run one per wavefront share the workgroup
-    // (kron_grouped phase 2), so they count 1/waves of their cost
+    // time of an entry in the one-launch kernel (kron_runs), in bytes of the
+    // workgroup's share of HBM bandwidth: its X, its flops at a quarter of a
+    // byte each, and a fixed term.  Large entries have their X copied in
+    // during the previous entry; small ones wait for their own loads (a
+    // latency of about 16 KiB at one workgroup's share) but run side by side
+    // on the workgroup's wavefronts.
     const int waves = std::max(1, k.nt / 64);
-    const double small = std::pow((double)nmax, d) / waves;
     auto cost = [&](int b) {
       const double sz = std::pow((double)n_host[b], d);
-      const double c = 2.0 * d * sz * n_host[b] + 4096.0;
-      return (waves > 1 && sz <= small) ? c / waves : c;
+      const double c = elem_bytes * sz + 0.5 * d * sz * n_host[b];
+      return (waves > 1 && sz <= small_max) ? (c + 16384.0) / waves : c + 2048.0;
     };
     const long long G = std::max(
         1LL, std::min<long long>((long long)device_cus() * occupancy(k.fn, k.nt),
@@ -1753,9 +1759,9 @@ extern "C"
     std::vector<int> ranges;
     ranges.push_back(0);
     int noutputs = 0;
+    // runs: entries of one output and one size
+    std::vector<int> starts;
     {
-      // runs: entries of one output and one size
-      std::vector<int> starts;
       std::vector<double> pre;
       double acc = 0;
       for (int i = 0; i < batchCount;)
@@ -1774,6 +1780,9 @@ extern "C"
       starts.push_back(batchCount);
       pre.push_back(acc);
       balanced_cuts(starts, pre, G, ranges);
+      // the cuts fall on run starts: express them in runs
+      for (int &c : ranges)
+        c = (int)(std::lower_bound(starts.begin(), starts.end(), c) - starts.begin());
     }
     p->ngroups = (int)ranges.size() - 1;
     p->noutputs = noutputs;
@@ -1837,6 +1846,9 @@ extern "C"
       KM_HIP(hipMalloc(&p->nvec, sizeof(int) * batchCount));
       KM_HIP(hipMalloc(&p->order, sizeof(int) * batchCount));
       KM_HIP(hipMalloc(&p->ranges, sizeof(int) * ranges.size()));
+      KM_HIP(hipMalloc(&p->runstart, sizeof(int) * starts.size()));
+      KM_HIP(hipMemcpy(p->runstart, starts.data(), sizeof(int) * starts.size(),
+                       hipMemcpyHostToDevice));
       KM_HIP(hipMemcpy(p->nvec, n_host, sizeof(int) * batchCount,
                        hipMemcpyHostToDevice));
       KM_HIP(hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
@@ -1887,6 +1899,8 @@ extern "C"
       (void)hipFree(plan->order);
     if (plan->ranges)
       (void)hipFree(plan->ranges);
+    if (plan->runstart)
+      (void)hipFree(plan->runstart);
     if (plan->binranges)
       (void)hipFree(plan->binranges);
     if (plan->ybind)

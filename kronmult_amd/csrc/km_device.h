@@ -17,7 +17,8 @@
 //              straight into per-thread registers;
 //   kron_sub + kron_pair  even one slab too large (d = 6, n = 7, 8): two
 //              passes through the caller's workspace;
-//   kron_grouped  entries of different n in one persistent launch;
+//   kron_runs  entries of different n in one launch, whole output runs per
+//              workgroup (km_runs.hip);
 //   kron_global  any shape; stages through the caller's workspace in global
 //              memory (the reference's own storage scheme) -- the fallback.
 // The first three need no global workspace.
@@ -1305,193 +1306,6 @@ __global__ __launch_bounds__(NT) void kron_slab_fwd(EntryArgs<T> a)
   if (p0 >= p1)
     return;
   slab_fwd_body<T, D, N, NT>(a, sb0, sb1, p0, p1);
-}
-
-// ---------------------------------------------------------------------------
-// kron_grouped: one persistent launch over entries of different sizes (square
-// n_b x n_b factors, n_b <= NMAX, all with the same d).  Entries arrive sorted
-// by size and output (a group plan), every workgroup walks its range and each
-// entry dispatches to the compile-time body of its n: the LDS-resident
-// algorithm of kron_lds with run merging of consecutive equal outputs.
-// ---------------------------------------------------------------------------
-template<int R, int NT, typename T>
-__device__ __forceinline__ void flush_dyn(T *dst, const T (&acc)[R], int tid,
-                                          int owned, int size)
-{
-  if (owned)
-  {
-    T yv[R];
-#pragma unroll
-    for (int r = 0; r < R; r++)
-    {
-      const int q = tid + NT * r;
-      if (q < size)
-        yv[r] = gld(dst + q);
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++)
-    {
-      const int q = tid + NT * r;
-      if (q < size)
-        gst(dst + q, yv[r] + acc[r]);
-    }
-  }
-  else
-  {
-#pragma unroll
-    for (int r = 0; r < R; r++)
-    {
-      const int q = tid + NT * r;
-      if (q < size)
-        gatomic_add(dst + q, acc[r]);
-    }
-  }
-}
-
-template<typename T, int D, int N, int NT, int R, bool EXACT, bool WAVE>
-__device__ __forceinline__ void grouped_entry(T *buf, const EntryArgs<T> &a,
-                                              int b, int tid, T (&acc)[R])
-{
-  constexpr int S = cpow(N, D);
-  static_assert(S <= R * NT, "accumulator too small for this size");
-  const T *xp = x_of(a, b);
-#pragma unroll
-  for (int r = 0; r < (S + NT - 1) / NT; r++)
-  {
-    const int q = tid + NT * r;
-    if (S % NT == 0 || q < S)
-      buf[q] = gld(xp + q);
-  }
-  pass_sync<WAVE>();
-  lds_stage<T, D, N, NT, EXACT, false, 0, WAVE>(buf, a, b, tid);
-#pragma unroll
-  for (int r = 0; r < (S + NT - 1) / NT; r++)
-  {
-    const int q = tid + NT * r;
-    if (S % NT == 0 || q < S)
-      acc[r] = acc[r] + buf[q];
-  }
-  pass_sync<WAVE>();
-}
-
-// dispatch one entry of size n to its compile-time body
-template<typename T, int D, int NMAX, int NT, int R, bool EXACT, bool WAVE>
-__device__ __forceinline__ void grouped_dispatch(int n, T *buf,
-                                                 const EntryArgs<T> &a, int b,
-                                                 int tid, T (&acc)[R])
-{
-  switch (n)
-  {
-#define KM_GROUPED_N(NN)                                                       \
-  case NN:                                                                     \
-    if constexpr (NN <= NMAX && cpow(NN, D) <= R * NT)                         \
-      grouped_entry<T, D, NN, NT, R, EXACT, WAVE>(buf, a, b, tid, acc);        \
-    break;
-    KM_GROUPED_N(1)
-    KM_GROUPED_N(2)
-    KM_GROUPED_N(3)
-    KM_GROUPED_N(4)
-    KM_GROUPED_N(5)
-    KM_GROUPED_N(6)
-    KM_GROUPED_N(7)
-    KM_GROUPED_N(8)
-#undef KM_GROUPED_N
-  default:
-    break;
-  }
-}
-
-// Entries of a workgroup's range come largest first (group plan order).
-// Phase 1: entries with n^d > SMALL use the whole workgroup.  Phase 2: the
-// remaining small entries are dealt out to the wavefronts output run by
-// output run (run j goes to wave j mod waves), each wavefront with its own
-// LDS region of SMALL elements and wave-level barriers, so a 16-element
-// entry does not idle 240 threads.  Runs stay whole, so outputs stay owned.
-template<typename T, int D, int NMAX, int NT, bool EXACT>
-__global__ __launch_bounds__(NT) void kron_grouped(EntryArgs<T> a,
-                                                   const int *nvec)
-{
-  plan_bind(a);
-  constexpr int SMAX = cpow(NMAX, D);
-  constexpr int NW = NT / 64;          // wavefronts per workgroup
-  constexpr int SMALL = SMAX / NW;     // elements per wavefront region
-  constexpr int R = (SMAX + NT - 1) / NT;
-  static_assert((SMALL + 63) / 64 <= R, "per-wave accumulator exceeds R");
-  __shared__ T buf[SMAX];
-  const int tid = threadIdx.x;
-  int p0, p1;
-  wg_range(a, p0, p1);
-  if (p0 >= p1)
-    return;
-  T acc[R];
-  T *target = nullptr;
-  int tsize = 0;
-  int p = p0;
-  // phase 1: large entries, whole workgroup
-  for (; p < p1; ++p)
-  {
-    const int b = entry_at(a, p);
-    const int n = cld(nvec + b);
-    int size = 1;
-    for (int k = 0; k < D; k++)
-      size *= n;
-    if (NW > 1 && size <= SMALL)
-      break;
-    T *yb = y_of(a, b);
-    if (yb != target)
-    {
-      if (target)
-        flush_dyn<R, NT>(target, acc, tid, a.owned, tsize);
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        acc[r] = 0;
-      target = yb;
-      tsize = size;
-    }
-    grouped_dispatch<T, D, NMAX, NT, R, EXACT, false>(n, buf, a, b, tid, acc);
-  }
-  if (target)
-    flush_dyn<R, NT>(target, acc, tid, a.owned, tsize);
-  if constexpr (NW > 1)
-  {
-    // phase 2: small entries, one output run per wavefront at a time
-    const int wave = tid >> 6, lane = tid & 63;
-    T *wbuf = buf + wave * SMALL;
-    T acc2[R];
-    T *t2 = nullptr;
-    int tsize2 = 0;
-    T *prev = nullptr;
-    int run = -1;
-    for (; p < p1; ++p)
-    {
-      const int b = entry_at(a, p);
-      T *yb = y_of(a, b);
-      if (yb != prev)
-      {
-        ++run;
-        prev = yb;
-      }
-      if (run % NW != wave)
-        continue;
-      const int n = cld(nvec + b);
-      int size = 1;
-      for (int k = 0; k < D; k++)
-        size *= n;
-      if (yb != t2)
-      {
-        if (t2)
-          flush_dyn<R, 64>(t2, acc2, lane, a.owned, tsize2);
-#pragma unroll
-        for (int r = 0; r < R; r++)
-          acc2[r] = 0;
-        t2 = yb;
-        tsize2 = size;
-      }
-      grouped_dispatch<T, D, NMAX, 64, R, EXACT, true>(n, wbuf, a, b, lane, acc2);
-    }
-    if (t2)
-      flush_dyn<R, 64>(t2, acc2, lane, a.owned, tsize2);
-  }
 }
 
 // ---------------------------------------------------------------------------

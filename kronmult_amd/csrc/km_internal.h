@@ -111,6 +111,12 @@ bool reg_kernel(int d, int n, bool exact, KernelInfo *out);
 template<typename T>
 bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
 
+// kron_runs (grouped variable-size batch in one launch, whole output runs
+// per workgroup) for dimension d: largest n in *nmax, largest entry size
+// (n^d) contracted by one wavefront in *small_max
+template<typename T>
+bool runs_kernel(int d, bool exact, KernelInfo *out, int *nmax, int *small_max);
+
 // kron_half (two workgroups per entry, split by the last mode's rows; FMA
 // calls) for (d, n) if one was built; pf = slabs per step (1 or 2)
 template<typename T>
@@ -144,10 +150,6 @@ hipError_t xcd_faults(unsigned long long *count);
 // KM_PRECISION_FP64_MFMA or KM_PRECISION_F16X4, if built for (d, n)
 template<typename T>
 bool twopass_mx_kernel(int d, int n, int precision, KernelInfo *sub);
-
-// kron_grouped for d (per-entry n up to *nmax)
-template<typename T>
-bool grouped_kernel(int d, bool exact, KernelInfo *out, int *nmax);
 
 // kron_global (always available)
 template<typename T>

@@ -1,0 +1,416 @@
+// km_runs.hip -- kron_runs: the grouped variable-size batch (per-entry n) in
+// one launch.  Each workgroup owns whole output runs (all entries of one
+// output, consecutive in the group plan's order: sizes descending, then
+// grouped by output), so every output is updated by one workgroup with a
+// plain read-modify-write, bitwise reproducible from call to call.
+//
+// Two ways through a run, chosen by the entry size S = n^d:
+//   large (S above one wavefront's LDS region): the whole workgroup
+//       contracts one entry at a time in LDS (one fiber per thread and mode),
+//       the run's sum kept in registers.  LDS holds two entries: while one is
+//       contracted, the next entry of the workgroup's range is copied in by
+//       direct-to-LDS loads and its factors are pulled into L2, so neither
+//       waits on global memory and no registers hold them.
+//   small: the run's entries are dealt out to the wavefronts (entry k of the
+//       run to wave k mod waves); each wave loads its entry into its own LDS
+//       region and contracts it with wave-level synchronisation only, summing
+//       its entries in registers, while its next entry's X and factors are
+//       pulled into L2.  At the end of the run the waves' partial sums meet
+//       in LDS and are added to Y in a fixed order.
+// A mode pass loads every fiber of a thread first and applies the factor a
+// few rows at a time from scalar registers (stage_apply), so n = 7, 8 need no
+// more SGPRs than one block of rows.  The entries of a run and the waves'
+// partial sums are added in a fixed order, so the result does not depend on
+// timing.  Mode order: d..1, the Alg. 993 backward order
+// (kronm_backward.hpp:92-139), as kron_lds.
+#include "km_internal.h"
+
+namespace km
+{
+
+template<typename T, int D, int NMAX, int NT>
+struct RunsGeom
+{
+  static constexpr int SMAX = cpow(NMAX, D);               // largest entry
+  static constexpr int NW = NT / 64;                        // wavefronts
+  static constexpr int LDSE = 2 * SMAX;                     // LDS elements
+  static constexpr int SW = NW > 1 ? (LDSE / NW) & ~3 : 0;  // per-wave region
+  static constexpr int RB = (SMAX + NT - 1) / NT;           // large-entry sum per thread
+  static constexpr int RS = NW > 1 ? (SW + 63) / 64 : 1;    // small-entry sum per lane
+};
+
+__device__ __forceinline__ int ipow_dev(int n, int d)
+{
+  int s = 1;
+  for (int k = 0; k < d; k++)
+    s *= n;
+  return s;
+}
+
+// Copy nelem elements at src into LDS at dst with direct-to-LDS loads:
+// 16 bytes per lane when src and dst are 16-byte aligned and the bytes are
+// whole 16-byte chunks, else 4 bytes.  Threads [0, NTH) take part (tid
+// relative to them: the workgroup or one wavefront); each wavefront writes
+// one contiguous block of LDS per instruction.  Completion is tracked by
+// vmcnt.
+template<typename T, int NTH>
+__device__ __forceinline__ void copy_to_lds(T *dst, const T *src, int nelem, int tid)
+{
+  const int bytes = nelem * (int)sizeof(T);
+  const int wave0 = (tid & ~63); // first thread of this wavefront
+  const int lane = tid & 63;
+  if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0 &&
+      (bytes & 15) == 0)
+  {
+    const int ch = bytes >> 4;
+    for (int c0 = wave0; c0 < ch; c0 += NTH)
+    {
+      const int c = c0 + lane;
+      if (c < ch)
+        __builtin_amdgcn_global_load_lds(
+            (gbl_vptr)(reinterpret_cast<const char *>(src) + (size_t)c * 16),
+            (lds_vptr)(reinterpret_cast<char *>(dst) + (size_t)c0 * 16), 16, 0, 0);
+    }
+  }
+  else
+  {
+    const int ch = bytes >> 2;
+    for (int c0 = wave0; c0 < ch; c0 += NTH)
+    {
+      const int c = c0 + lane;
+      if (c < ch)
+        __builtin_amdgcn_global_load_lds(
+            (gbl_vptr)(reinterpret_cast<const char *>(src) + (size_t)c * 4),
+            (lds_vptr)(reinterpret_cast<char *>(dst) + (size_t)c0 * 4), 4, 0, 0);
+    }
+  }
+}
+
+// Pull lines into L2 without registers: direct-to-LDS loads (4 bytes per
+// lane, one lane per 128-byte line) into a scratch block of LDS that nothing
+// reads.  Lanes [0, 64) of one wavefront; elems elements at p.
+template<typename T>
+__device__ __forceinline__ void pull_lines(const T *p, int elems, int lane, int *scratch)
+{
+  constexpr int PL = 128 / (int)sizeof(T);
+  const int lines = (elems + PL - 1) / PL;
+  for (int l0 = 0; l0 < lines; l0 += 64)
+    if (l0 + lane < lines)
+      __builtin_amdgcn_global_load_lds(
+          (gbl_vptr)(p + (size_t)(l0 + lane) * PL), (lds_vptr)(scratch), 4, 0, 0);
+}
+
+// the d factors of entry b (n x n each, stored densely) into L2
+template<typename T, int D>
+__device__ __forceinline__ void pull_a(const EntryArgs<T> &a, int b, int n, int lane,
+                                       int *scratch)
+{
+#pragma unroll
+  for (int k = 0; k < D; k++)
+    pull_lines(a_of(a, b, k, D), n * n, lane, scratch);
+}
+
+// Y(0:S) += acc, thread tid holding elements tid + NTH*r: plain
+// read-modify-write for an owned output, atomics otherwise.
+template<int R, int NTH, typename T>
+__device__ __forceinline__ void runs_flush(T *dst, const T (&acc)[R], int tid,
+                                           int S, int owned)
+{
+  if (owned)
+  {
+    T yv[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NTH * r;
+      if (q < S)
+        yv[r] = gld(dst + q);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NTH * r;
+      if (q < S)
+        gst(dst + q, yv[r] + acc[r]);
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+      const int q = tid + NTH * r;
+      if (q < S)
+        gatomic_add(dst + q, acc[r]);
+    }
+  }
+}
+
+// modes d..1 of the entry in buf (N^D elements, LDS) with NTH threads
+template<typename T, int D, int N, int NTH, bool EXACT, int K, bool WAVE>
+__device__ __forceinline__ void runs_stages(T *buf, const EntryArgs<T> &a, int b, int tid)
+{
+  constexpr int MODE = D - 1 - K;
+  constexpr int STRIDE = cpow(N, D - 1 - MODE);
+  constexpr int F = cpow(N, D) / N;
+  stage_apply<T, N, NTH, EXACT, F, STRIDE>(buf, a_of(a, b, MODE, D), N, tid);
+  pass_sync<WAVE>();
+  if constexpr (K + 1 < D)
+    runs_stages<T, D, N, NTH, EXACT, K + 1, WAVE>(buf, a, b, tid);
+}
+
+// contract the entry in buf (size N^D) with NTH threads and add it to acc
+template<typename T, int D, int N, int NTH, int R, bool EXACT, bool WAVE>
+__device__ __forceinline__ void runs_entry(T *buf, const EntryArgs<T> &a, int b, int tid,
+                                           T (&acc)[R])
+{
+  constexpr int S = cpow(N, D);
+  runs_stages<T, D, N, NTH, EXACT, 0, WAVE>(buf, a, b, tid);
+#pragma unroll
+  for (int r = 0; r < (S + NTH - 1) / NTH; r++)
+  {
+    const int q = tid + NTH * r;
+    if (S % NTH == 0 || q < S)
+      acc[r] = acc[r] + buf[q];
+  }
+}
+
+// dispatch an entry of size n to its compile-time body: small bodies (the
+// entry fits one wavefront's region) with one wavefront, large ones with the
+// workgroup
+template<typename T, int D, int NMAX, int NT, bool EXACT, bool LARGE, int R>
+__device__ __forceinline__ void runs_dispatch(int n, T *buf, const EntryArgs<T> &a, int b,
+                                              int tid, T (&acc)[R])
+{
+  using G = RunsGeom<T, D, NMAX, NT>;
+  constexpr int NTH = LARGE ? NT : 64;
+  switch (n)
+  {
+#define KM_RUNS_N(NN)                                                          \
+  case NN:                                                                     \
+    if constexpr (NN <= NMAX && (LARGE ? (G::NW == 1 || cpow(NN, D) > G::SW)   \
+                                       : (G::NW > 1 && cpow(NN, D) <= G::SW))) \
+      runs_entry<T, D, NN, NTH, R, EXACT, !LARGE>(buf, a, b, tid, acc);        \
+    break;
+    KM_RUNS_N(1)
+    KM_RUNS_N(2)
+    KM_RUNS_N(3)
+    KM_RUNS_N(4)
+    KM_RUNS_N(5)
+    KM_RUNS_N(6)
+    KM_RUNS_N(7)
+    KM_RUNS_N(8)
+#undef KM_RUNS_N
+  default:
+    break;
+  }
+}
+
+// Kernel arguments beyond EntryArgs: nvec[b] = n of entry b; runstart[r] =
+// first order position of output run r (runs of one output and one size,
+// sizes descending); the workgroup's ranges are in runs.
+template<typename T, int D, int NMAX, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, const int *nvec,
+                                                const int *runstart)
+{
+  using G = RunsGeom<T, D, NMAX, NT>;
+  plan_bind(a);
+  __shared__ __attribute__((aligned(16))) T lds[G::LDSE];
+  __shared__ int scratch[64];
+  const int tid = threadIdx.x;
+  int r0, r1;
+  wg_range(a, r0, r1);
+  if (r0 >= r1)
+    return;
+  auto small = [&](int n) { return G::NW > 1 && ipow_dev(n, D) <= G::SW; };
+
+  // large runs come first (sizes descend along the plan's order)
+  int rb = r0;
+  while (rb < r1 && !small(cld(nvec + entry_at(a, cld(runstart + rb)))))
+    ++rb;
+
+  if (rb > r0)
+  {
+    const int q0 = cld(runstart + r0), qb = cld(runstart + rb);
+    T acc[G::RB];
+#pragma unroll
+    for (int r = 0; r < G::RB; r++)
+      acc[r] = 0;
+    T *target = nullptr;
+    int tn = 0, tsize = 0;
+    int b = entry_at(a, q0);
+    int n = cld(nvec + b);
+    copy_to_lds<T, NT>(lds, x_of(a, b), ipow_dev(n, D), tid);
+    if (tid < 64)
+      pull_a<T, D>(a, b, n, tid, scratch);
+    for (int p = q0; p < qb; ++p)
+    {
+      // the next entry's pointers, loaded while this entry waits
+      int bn = b, nn = n;
+      const T *xn = nullptr;
+      if (p + 1 < qb)
+      {
+        bn = entry_at(a, p + 1);
+        nn = cld(nvec + bn);
+        xn = x_of(a, bn);
+      }
+      // offsets, not a select of two pointers: a select lets the compiler
+      // turn every LDS read into reads of both buffers and a select
+      const int par = (p - q0) & 1;
+      T *cur = lds + par * G::SMAX;
+      T *nxt = lds + (par ^ 1) * G::SMAX;
+      T *yb = y_of(a, b);
+      if (yb != target || n != tn)
+      {
+        if (target)
+          runs_flush<G::RB, NT>(target, acc, tid, tsize, a.owned);
+#pragma unroll
+        for (int r = 0; r < G::RB; r++)
+          acc[r] = 0;
+        target = yb;
+        tn = n;
+        tsize = ipow_dev(n, D);
+      }
+      // this thread's copies into cur have landed; after the barrier all
+      // have, and every thread is done with nxt (the previous entry)
+      __builtin_amdgcn_s_waitcnt(0);
+      lds_barrier();
+      if (p + 1 < qb)
+      {
+        // the next entry: X into the other buffer, its factors into L2
+        copy_to_lds<T, NT>(nxt, xn, ipow_dev(nn, D), tid);
+        if (tid < 64)
+          pull_a<T, D>(a, bn, nn, tid, scratch);
+      }
+      runs_dispatch<T, D, NMAX, NT, EXACT, true>(n, cur, a, b, tid, acc);
+      b = bn;
+      n = nn;
+    }
+    runs_flush<G::RB, NT>(target, acc, tid, tsize, a.owned);
+  }
+
+  if constexpr (G::NW > 1)
+  {
+    // the wavefront index is uniform over the wavefront: made explicit, so
+    // everything derived from it (entries, pointers, factors) stays scalar
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    T *wbuf = lds + wave * G::SW;
+    for (int r = rb; r < r1; r++)
+    {
+      // one output run [p, re): entries of one output and one size
+      const int p = cld(runstart + r), re = cld(runstart + r + 1);
+      const int b0 = entry_at(a, p);
+      T *yb = y_of(a, b0);
+      const int n = cld(nvec + b0);
+      const int S = ipow_dev(n, D);
+      // the regions are free: every wave is done with the previous run
+      lds_barrier();
+      T acc[G::RS];
+#pragma unroll
+      for (int q = 0; q < G::RS; q++)
+        acc[q] = 0;
+      int e = p + wave;
+      int b = e < re ? entry_at(a, e) : 0;
+      const T *xp = e < re ? x_of(a, b) : nullptr;
+      for (; e < re; e += G::NW)
+      {
+        copy_to_lds<T, 64>(wbuf, xp, S, lane);
+        // this wave's next entry of the run: pointers, its X and factors
+        // into L2
+        int bn = b;
+        const T *xn = nullptr;
+        if (e + G::NW < re)
+        {
+          bn = entry_at(a, e + G::NW);
+          xn = x_of(a, bn);
+          pull_lines(xn, S, lane, scratch);
+          pull_a<T, D>(a, bn, n, lane, scratch);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        pass_sync<true>();
+        runs_dispatch<T, D, NMAX, NT, EXACT, false>(n, wbuf, a, b, lane, acc);
+        pass_sync<true>(); // the region is read before it is reloaded
+        if (!a.owned)
+        {
+          // the caller's outputs differ from the plan's (plan_bind): each
+          // entry goes to its own output with atomics
+          runs_flush<G::RS, 64>(y_of(a, b), acc, lane, S, 0);
+#pragma unroll
+          for (int q = 0; q < G::RS; q++)
+            acc[q] = 0;
+        }
+        b = bn;
+        xp = xn;
+      }
+      if (!a.owned)
+        continue;
+      const int used = min(G::NW, re - p);
+      if (wave < used)
+      {
+#pragma unroll
+        for (int q = 0; q < G::RS; q++)
+        {
+          const int i = lane + 64 * q;
+          if (i < S)
+            wbuf[i] = acc[q];
+        }
+      }
+      lds_barrier();
+      // fixed order: wave 0's partial first, then 1, ..
+      constexpr int RT = (G::SW + NT - 1) / NT;
+      T sum[RT];
+#pragma unroll
+      for (int q = 0; q < RT; q++)
+      {
+        const int i = tid + NT * q;
+        sum[q] = 0;
+        if (i < S)
+        {
+          T v = lds[i];
+          for (int w = 1; w < used; w++)
+            v = v + lds[w * G::SW + i];
+          sum[q] = v;
+        }
+      }
+      runs_flush<RT, NT>(yb, sum, tid, S, a.owned);
+    }
+  }
+}
+
+#define KM_RUNS_CASE(T, D, NMAX, NT)                                           \
+  if (d == D)                                                                  \
+  {                                                                            \
+    out->fn = exact ? (const void *)&kron_runs<T, D, NMAX, NT, true>           \
+                    : (const void *)&kron_runs<T, D, NMAX, NT, false>;         \
+    out->nt = NT;                                                              \
+    out->family = FAM_GROUPED;                                                 \
+    *nmax = NMAX;                                                              \
+    *small_max = RunsGeom<T, D, NMAX, NT>::SW;                                 \
+    return true;                                                               \
+  }
+
+template<>
+bool runs_kernel<double>(int d, bool exact, KernelInfo *out, int *nmax, int *small_max)
+{
+  KM_RUNS_CASE(double, 1, 8, 64)
+  KM_RUNS_CASE(double, 2, 8, 64)
+  KM_RUNS_CASE(double, 3, 8, 256)
+  KM_RUNS_CASE(double, 4, 8, 512)
+  KM_RUNS_CASE(double, 5, 6, 512)
+  KM_RUNS_CASE(double, 6, 4, 512)
+  return false;
+}
+
+template<>
+bool runs_kernel<float>(int d, bool exact, KernelInfo *out, int *nmax, int *small_max)
+{
+  KM_RUNS_CASE(float, 1, 8, 64)
+  KM_RUNS_CASE(float, 2, 8, 64)
+  KM_RUNS_CASE(float, 3, 8, 256)
+  KM_RUNS_CASE(float, 4, 8, 512)
+  KM_RUNS_CASE(float, 5, 6, 512)
+  KM_RUNS_CASE(float, 6, 4, 512)
+  return false;
+}
+
+} // namespace km
