@@ -85,6 +85,22 @@ def test_grouped_vs_oracle(km, d, sizes, mode):
             assert np.array_equal(outs[0][n], outs[1][n])
 
 
+def test_grouped_persistent_long_runs_both_paths(km):
+    """One launch over long output runs of both kinds of entry: d = 4 with
+    n = 8 (a whole workgroup per entry, next X copied in behind it) and n = 3
+    (entries dealt out to the wavefronts, partial sums combined in LDS), 40
+    entries per output, so every wave takes several entries of one run;
+    bitwise equal from call to call."""
+    d, sizes = 4, [3, 8]
+    ns, yg, mats, X, Y0 = _make(91, d, sizes, 160, 2)
+    want = _oracle(d, ns, yg, mats, X, Y0)
+    outs, (nout, _, _) = _gpu(km, d, ns, yg, mats, X, Y0, runs=2, mode="persistent")
+    assert nout == 4
+    for n in want:
+        assert rel_err(outs[0][n], want[n]) <= RTOL_FP64, n
+        assert np.array_equal(outs[0][n], outs[1][n])
+
+
 def test_grouped_large_bins_vs_oracle(km):
     """Bins larger than one resident wave of workgroups, so the even split
     cuts output runs and the atomic flush of a shared run is exercised."""
@@ -124,7 +140,7 @@ def test_grouped_rejects_oversize(km):
     assert ei.value.status == km.ERROR_NOT_SUPPORTED
 
 
-@pytest.mark.parametrize("mode", ["atomic", "deterministic"])
+@pytest.mark.parametrize("mode", ["atomic", "deterministic", "persistent"])
 def test_grouped_graph_capture_replays(km, mode):
     """The per-size launches fork onto the plan's streams and join back into
     the caller's stream; captured in a HIP graph (torch.cuda.graph) and
@@ -159,7 +175,7 @@ def test_grouped_graph_capture_replays(km, mode):
         assert junk.item() == float(1 << 20)
         for n in tY:
             got = tY[n] - tY0[n]
-            if mode == "deterministic":
+            if mode != "atomic":
                 assert torch.equal(got, eager[n]), (rep, n)
             else:
                 assert torch.allclose(got, eager[n], rtol=0, atol=1e-12 * float(eager[n].abs().max())), (rep, n)
