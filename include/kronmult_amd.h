@@ -271,7 +271,12 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
  * through the fork (a per-plan lock).  KM_FLAG_PERSISTENT runs all bins in
  * one launch with owned outputs (each workgroup takes whole output runs;
  * bitwise reproducible; fastest of the owned modes when small sizes
- * dominate).  Creating the plan reads the host array n_host
+ * dominate).  An output run of large entries too long for one workgroup is
+ * cut into parts whose sums the plan keeps in its own device buffer and a
+ * second short launch adds to the output in part order, so such a call is
+ * two launches, and persistent calls of one plan must not overlap (on
+ * several streams) while the plan has such runs (num_groups counts the
+ * first launch's workgroups).  Creating the plan reads the host array n_host
  * and the device pointer array pY once.  The owned modes are bound to those
  * pY contents as output-ordered plans are (km_plan_create): a call whose pY
  * differs accumulates with atomics, and KM_FLAG_PLAN_OUTPUTS writes to the

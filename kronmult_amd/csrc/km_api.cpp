@@ -62,10 +62,21 @@ struct km_group_plan_s
   int noutputs = 0;
   int *nvec = nullptr;   // device, per entry
   int *order = nullptr;  // device, batch entries
-  // one-launch kernel (kron_runs): workgroup w takes output runs
-  // [ranges[w], ranges[w+1]); run r starts at order position runstart[r]
+  // one-launch kernel (kron_runs): workgroup w takes work units
+  // [ranges[w], ranges[w+1]); unit u covers order positions
+  // [runstart[u], runstart[u+1]): a whole output run, or one part of a long
+  // run of large entries split over several workgroups (unitmeta[3u..3u+2]
+  // = split id, part, parts; split id -1 for a whole run).  Part sums of
+  // split run s go to splitws + splitoff[s] + part*n^d; a second launch
+  // (kron_runs_combine) adds them to the output in part order.
   int *ranges = nullptr;   // device, ngroups + 1 entries
-  int *runstart = nullptr; // device, noutputs + 1 entries
+  int *runstart = nullptr; // device, units + 1 entries
+  int *unitmeta = nullptr; // device, 3 per unit
+  void *splitws = nullptr; // device, partial sums of split runs
+  long long *splitoff = nullptr; // device, per split run
+  int *splitfirst = nullptr; // device, per split run: first order position
+  int *splitparts = nullptr; // device, per split run: parts
+  int nsplit = 0;
   // per-size-bin launch: bin b uses binranges[first .. first+count]
   std::vector<km_group_bin> bins;
   int *binranges = nullptr; // device
@@ -1124,11 +1135,27 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
   a.owned = 1;
   a.bind_flag = bind_flag;
   a.bind_gen = bind_gen;
-  const int *nv = plan->nvec;
-  const int *rs = plan->runstart;
-  void *args[] = {&a, &nv, &rs};
+  km::RunsArgs<T> ra;
+  ra.nvec = plan->nvec;
+  ra.ustart = plan->runstart;
+  ra.umeta = plan->unitmeta;
+  ra.ws = static_cast<T *>(plan->splitws);
+  ra.wsoff = plan->splitoff;
+  ra.sfirst = plan->splitfirst;
+  ra.sparts = plan->splitparts;
+  void *args[] = {&a, &ra};
   KM_HIP(hipLaunchKernel(k.fn, dim3(plan->ngroups), dim3(k.nt), args, 0,
                          (hipStream_t)stream));
+  if (plan->nsplit > 0)
+  {
+    // the parts of split runs, added to their outputs in part order
+    int cnt_nt = 0;
+    const void *cf = km::runs_combine_kernel<T>(plan->d, &cnt_nt);
+    if (!cf)
+      return KM_ERROR_NOT_SUPPORTED;
+    KM_HIP(hipLaunchKernel(cf, dim3(plan->nsplit), dim3(cnt_nt), args, 0,
+                           (hipStream_t)stream));
+  }
   return KM_SUCCESS;
 }
 
@@ -1741,46 +1768,103 @@ extern "C"
         return n_host[x] > n_host[y];
       return ys[x] < ys[y];
     });
-    // time of an entry in the one-launch kernel (kron_runs), in bytes of the
-    // workgroup's share of HBM bandwidth: its X, its flops at a quarter of a
-    // byte each, and a fixed term.  Large entries have their X copied in
-    // during the previous entry; small ones wait for their own loads (a
-    // latency of about 16 KiB at one workgroup's share) but run side by side
-    // on the workgroup's wavefronts.
+    // Time of an entry in the one-launch kernel (kron_runs), in units of one
+    // byte at a workgroup's share of HBM bandwidth.  c = its X bytes plus its
+    // flops at a quarter of a byte each.  Large entries have their X copied
+    // in during the previous entry: c plus a small fixed term.  Small ones
+    // run side by side on the workgroup's wavefronts but each waits on its
+    // own loads and factor reads: (39,300 + 7 c) / waves, fitted to measured
+    // d = 4 runs (n = 2 and n = 5 cost 0.39 and 1.1 us per entry where n = 8
+    // costs 7.5).  Every run adds the fixed cost of its flush.
     const int waves = std::max(1, k.nt / 64);
     auto cost = [&](int b) {
       const double sz = std::pow((double)n_host[b], d);
       const double c = elem_bytes * sz + 0.5 * d * sz * n_host[b];
-      return (waves > 1 && sz <= small_max) ? (c + 16384.0) / waves : c + 2048.0;
+      return (waves > 1 && sz <= small_max) ? (39300.0 + 7.0 * c) / waves : c + 2048.0;
     };
+    constexpr double kRunCost = 25000.0;
+    // A run of large entries costing more than kSplitShares workgroup shares
+    // is cut into at most kSplitParts parts of about that size.  Measured on
+    // d = 4, 20000 entries: 285 outputs of n = 8 only 530 -> 497 us (a
+    // threshold of half a share: 397 us, but the 2..8 mix, whose runs are
+    // shorter, 223 -> 240 us from the part stores and the combine launch).
+    constexpr double kSplitShares = 2.0;
+    constexpr int kSplitParts = 16;
     const long long G = std::max(
         1LL, std::min<long long>((long long)device_cus() * occupancy(k.fn, k.nt),
                                  batchCount));
     std::vector<int> ranges;
     ranges.push_back(0);
     int noutputs = 0;
-    // runs: entries of one output and one size
-    std::vector<int> starts;
+    // runs: entries of one output and one size; then work units: a run of
+    // large entries (whole-workgroup body) that would cost more than
+    // kSplitShares workgroup shares is cut into parts that workgroups take
+    // separately; a second launch adds their sums to the output in part
+    // order
+    std::vector<int> starts;    // unit starts (order positions)
+    std::vector<int> unitmeta;  // split id, part, parts
+    std::vector<long long> splitoff;
+    std::vector<int> splitfirst, splitparts;
+    long long split_elems = 0;
     {
-      std::vector<double> pre;
-      double acc = 0;
+      std::vector<int> rstart;
+      std::vector<double> rcost;
+      double total = 0;
       for (int i = 0; i < batchCount;)
       {
         int j = i + 1;
         while (j < batchCount && ys[order[j]] == ys[order[i]] &&
                n_host[order[j]] == n_host[order[i]])
           j++;
-        starts.push_back(i);
-        pre.push_back(acc);
+        double c = kRunCost;
         for (int q = i; q < j; q++)
-          acc += cost(order[q]);
+          c += cost(order[q]);
+        rstart.push_back(i);
+        rcost.push_back(c);
+        total += c;
         noutputs++;
         i = j;
+      }
+      rstart.push_back(batchCount);
+      const double share = total / (double)G;
+      std::vector<double> pre;
+      double acc = 0;
+      for (size_t r = 0; r + 1 < rstart.size(); r++)
+      {
+        const int i = rstart[r], j = rstart[r + 1], k = j - i;
+        const int n = n_host[order[i]];
+        const double sz = std::pow((double)n, d);
+        const bool large = !(waves > 1 && sz <= small_max);
+        int parts = 1;
+        if (large && k > 1 && rcost[r] > kSplitShares * share)
+          parts = (int)std::min<double>(
+              {(double)k, (double)kSplitParts, std::ceil(rcost[r] / (kSplitShares * share))});
+        for (int q = 0; q < parts; q++)
+        {
+          const int us = i + (int)((long long)k * q / parts);
+          const int ue = i + (int)((long long)k * (q + 1) / parts);
+          starts.push_back(us);
+          pre.push_back(acc);
+          double c = kRunCost / parts;
+          for (int e = us; e < ue; e++)
+            c += cost(order[e]);
+          acc += c;
+          unitmeta.push_back(parts > 1 ? (int)splitoff.size() : -1);
+          unitmeta.push_back(q);
+          unitmeta.push_back(parts);
+        }
+        if (parts > 1)
+        {
+          splitoff.push_back(split_elems);
+          splitfirst.push_back(i);
+          splitparts.push_back(parts);
+          split_elems += (long long)parts * (long long)sz;
+        }
       }
       starts.push_back(batchCount);
       pre.push_back(acc);
       balanced_cuts(starts, pre, G, ranges);
-      // the cuts fall on run starts: express them in runs
+      // the cuts fall on unit starts: express them in units
       for (int &c : ranges)
         c = (int)(std::lower_bound(starts.begin(), starts.end(), c) - starts.begin());
     }
@@ -1849,6 +1933,23 @@ extern "C"
       KM_HIP(hipMalloc(&p->runstart, sizeof(int) * starts.size()));
       KM_HIP(hipMemcpy(p->runstart, starts.data(), sizeof(int) * starts.size(),
                        hipMemcpyHostToDevice));
+      KM_HIP(hipMalloc(&p->unitmeta, sizeof(int) * unitmeta.size()));
+      KM_HIP(hipMemcpy(p->unitmeta, unitmeta.data(), sizeof(int) * unitmeta.size(),
+                       hipMemcpyHostToDevice));
+      if (!splitoff.empty())
+      {
+        KM_HIP(hipMalloc(&p->splitws, (size_t)elem_bytes * (size_t)split_elems));
+        KM_HIP(hipMalloc(&p->splitoff, sizeof(long long) * splitoff.size()));
+        KM_HIP(hipMemcpy(p->splitoff, splitoff.data(), sizeof(long long) * splitoff.size(),
+                         hipMemcpyHostToDevice));
+        KM_HIP(hipMalloc(&p->splitfirst, sizeof(int) * splitfirst.size()));
+        KM_HIP(hipMemcpy(p->splitfirst, splitfirst.data(), sizeof(int) * splitfirst.size(),
+                         hipMemcpyHostToDevice));
+        KM_HIP(hipMalloc(&p->splitparts, sizeof(int) * splitparts.size()));
+        KM_HIP(hipMemcpy(p->splitparts, splitparts.data(), sizeof(int) * splitparts.size(),
+                         hipMemcpyHostToDevice));
+        p->nsplit = (int)splitoff.size();
+      }
       KM_HIP(hipMemcpy(p->nvec, n_host, sizeof(int) * batchCount,
                        hipMemcpyHostToDevice));
       KM_HIP(hipMemcpy(p->order, order.data(), sizeof(int) * batchCount,
@@ -1901,6 +2002,16 @@ extern "C"
       (void)hipFree(plan->ranges);
     if (plan->runstart)
       (void)hipFree(plan->runstart);
+    if (plan->unitmeta)
+      (void)hipFree(plan->unitmeta);
+    if (plan->splitws)
+      (void)hipFree(plan->splitws);
+    if (plan->splitoff)
+      (void)hipFree(plan->splitoff);
+    if (plan->splitfirst)
+      (void)hipFree(plan->splitfirst);
+    if (plan->splitparts)
+      (void)hipFree(plan->splitparts);
     if (plan->binranges)
       (void)hipFree(plan->binranges);
     if (plan->ybind)

@@ -111,6 +111,28 @@ bool reg_kernel(int d, int n, bool exact, KernelInfo *out);
 template<typename T>
 bool slab_kernel(int d, int n, bool exact, KernelInfo *out);
 
+// Arguments of kron_runs and kron_runs_combine beyond EntryArgs (all device
+// memory, from the group plan): nvec[b] = n of entry b; work unit u covers
+// order positions [ustart[u], ustart[u+1]), a whole output run or one part
+// of a split run; umeta[3u..3u+2] = split id (-1: whole run), part, parts;
+// split run s keeps its part sums at ws + wsoff[s] + part*n^d, and its
+// entries start at order position sfirst[s] (sparts[s] parts).
+template<typename T>
+struct RunsArgs
+{
+  const int *nvec;
+  const int *ustart;
+  const int *umeta;
+  T *ws;
+  const long long *wsoff;
+  const int *sfirst;
+  const int *sparts;
+};
+
+// kron_runs_combine: adds the part sums of every split run to its output
+template<typename T>
+const void *runs_combine_kernel(int d, int *nt);
+
 // kron_runs (grouped variable-size batch in one launch, whole output runs
 // per workgroup) for dimension d: largest n in *nmax, largest entry size
 // (n^d) contracted by one wavefront in *small_max

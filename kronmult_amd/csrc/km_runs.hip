@@ -206,41 +206,92 @@ __device__ __forceinline__ void runs_dispatch(int n, T *buf, const EntryArgs<T> 
   }
 }
 
-// Kernel arguments beyond EntryArgs: nvec[b] = n of entry b; runstart[r] =
-// first order position of output run r (runs of one output and one size,
-// sizes descending); the workgroup's ranges are in runs.
+// Part sum of a split run: stored to the plan's workspace; kron_runs_combine,
+// launched right after on the same stream, adds the parts to Y in part order.
+template<int R, int NT, typename T>
+__device__ __forceinline__ void runs_part_store(const RunsArgs<T> &ra, int sid, int part,
+                                                const T (&acc)[R], int tid, int S)
+{
+  T *mine = ra.ws + cld(ra.wsoff + sid) + (long long)part * S;
+#pragma unroll
+  for (int r = 0; r < R; r++)
+  {
+    const int q = tid + NT * r;
+    if (q < S)
+      gst(mine + q, acc[r]);
+  }
+}
+
+// One workgroup per split run: Y += part 0 + part 1 + ... (in that order),
+// so the result does not depend on which part finished first.
+template<typename T, int D, int NT>
+__global__ __launch_bounds__(NT) void kron_runs_combine(EntryArgs<T> a, RunsArgs<T> ra)
+{
+  plan_bind(a);
+  if (!a.owned)
+    return; // the caller's outputs differ from the plan's: parts went to Y with atomics
+  const int sid = blockIdx.x;
+  const int b = entry_at(a, cld(ra.sfirst + sid));
+  const int S = ipow_dev(cld(ra.nvec + b), D);
+  const int parts = cld(ra.sparts + sid);
+  const T *base = ra.ws + cld(ra.wsoff + sid);
+  T *y = y_of(a, b);
+  for (int q = threadIdx.x; q < S; q += NT)
+  {
+    T v = gld(base + q);
+    for (int k = 1; k < parts; k++)
+      v = v + gld(base + (long long)k * S + q);
+    gst(y + q, gld(y + q) + v);
+  }
+}
+
 template<typename T, int D, int NMAX, int NT, bool EXACT>
-__global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, const int *nvec,
-                                                const int *runstart)
+__global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, RunsArgs<T> ra)
 {
   using G = RunsGeom<T, D, NMAX, NT>;
   plan_bind(a);
   __shared__ __attribute__((aligned(16))) T lds[G::LDSE];
   __shared__ int scratch[64];
   const int tid = threadIdx.x;
-  int r0, r1;
-  wg_range(a, r0, r1);
-  if (r0 >= r1)
+  int u0, u1;
+  wg_range(a, u0, u1);
+  if (u0 >= u1)
     return;
+  const int *nvec = ra.nvec;
   auto small = [&](int n) { return G::NW > 1 && ipow_dev(n, D) <= G::SW; };
 
-  // large runs come first (sizes descend along the plan's order)
-  int rb = r0;
-  while (rb < r1 && !small(cld(nvec + entry_at(a, cld(runstart + rb)))))
-    ++rb;
+  // units of large entries come first (sizes descend along the plan's order)
+  int ub = u0;
+  while (ub < u1 && !small(cld(nvec + entry_at(a, cld(ra.ustart + ub)))))
+    ++ub;
 
-  if (rb > r0)
+  if (ub > u0)
   {
-    const int q0 = cld(runstart + r0), qb = cld(runstart + rb);
+    const int q0 = cld(ra.ustart + u0), qb = cld(ra.ustart + ub);
     T acc[G::RB];
 #pragma unroll
     for (int r = 0; r < G::RB; r++)
       acc[r] = 0;
-    T *target = nullptr;
-    int tn = 0, tsize = 0;
     int b = entry_at(a, q0);
     int n = cld(nvec + b);
-    copy_to_lds<T, NT>(lds, x_of(a, b), ipow_dev(n, D), tid);
+    // the unit (or, when the caller's outputs differ from the plan's, the
+    // output run) being summed
+    T *target = y_of(a, b);
+    int tn = n, tsize = ipow_dev(n, D);
+    int cu = u0, ce = cld(ra.ustart + u0 + 1);
+    auto flush = [&]() {
+      if (!a.owned)
+      {
+        runs_flush<G::RB, NT>(target, acc, tid, tsize, 0);
+        return;
+      }
+      const int sid = cld(ra.umeta + 3 * cu);
+      if (sid < 0)
+        runs_flush<G::RB, NT>(target, acc, tid, tsize, 1);
+      else
+        runs_part_store<G::RB, NT>(ra, sid, cld(ra.umeta + 3 * cu + 1), acc, tid, tsize);
+    };
+    copy_to_lds<T, NT>(lds, x_of(a, b), tsize, tid);
     if (tid < 64)
       pull_a<T, D>(a, b, n, tid, scratch);
     for (int p = q0; p < qb; ++p)
@@ -259,14 +310,22 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, const int *nvec,
       const int par = (p - q0) & 1;
       T *cur = lds + par * G::SMAX;
       T *nxt = lds + (par ^ 1) * G::SMAX;
-      T *yb = y_of(a, b);
-      if (yb != target || n != tn)
+      if (p > q0)
       {
-        if (target)
-          runs_flush<G::RB, NT>(target, acc, tid, tsize, a.owned);
+        T *yb = y_of(a, b);
+        const bool unit_end = p == ce;
+        if (a.owned ? unit_end : (yb != target || n != tn))
+        {
+          flush();
 #pragma unroll
-        for (int r = 0; r < G::RB; r++)
-          acc[r] = 0;
+          for (int r = 0; r < G::RB; r++)
+            acc[r] = 0;
+        }
+        if (unit_end)
+        {
+          ++cu;
+          ce = cld(ra.ustart + cu + 1);
+        }
         target = yb;
         tn = n;
         tsize = ipow_dev(n, D);
@@ -286,7 +345,7 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, const int *nvec,
       b = bn;
       n = nn;
     }
-    runs_flush<G::RB, NT>(target, acc, tid, tsize, a.owned);
+    flush();
   }
 
   if constexpr (G::NW > 1)
@@ -295,10 +354,10 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, const int *nvec,
     // everything derived from it (entries, pointers, factors) stays scalar
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     T *wbuf = lds + wave * G::SW;
-    for (int r = rb; r < r1; r++)
+    for (int r = ub; r < u1; r++)
     {
       // one output run [p, re): entries of one output and one size
-      const int p = cld(runstart + r), re = cld(runstart + r + 1);
+      const int p = cld(ra.ustart + r), re = cld(ra.ustart + r + 1);
       const int b0 = entry_at(a, p);
       T *yb = y_of(a, b0);
       const int n = cld(nvec + b0);
@@ -388,6 +447,38 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, const int *nvec,
     *small_max = RunsGeom<T, D, NMAX, NT>::SW;                                 \
     return true;                                                               \
   }
+
+template<>
+const void *runs_combine_kernel<double>(int d, int *nt)
+{
+  *nt = 256;
+  switch (d)
+  {
+  case 1: return (const void *)&kron_runs_combine<double, 1, 256>;
+  case 2: return (const void *)&kron_runs_combine<double, 2, 256>;
+  case 3: return (const void *)&kron_runs_combine<double, 3, 256>;
+  case 4: return (const void *)&kron_runs_combine<double, 4, 256>;
+  case 5: return (const void *)&kron_runs_combine<double, 5, 256>;
+  case 6: return (const void *)&kron_runs_combine<double, 6, 256>;
+  default: return nullptr;
+  }
+}
+
+template<>
+const void *runs_combine_kernel<float>(int d, int *nt)
+{
+  *nt = 256;
+  switch (d)
+  {
+  case 1: return (const void *)&kron_runs_combine<float, 1, 256>;
+  case 2: return (const void *)&kron_runs_combine<float, 2, 256>;
+  case 3: return (const void *)&kron_runs_combine<float, 3, 256>;
+  case 4: return (const void *)&kron_runs_combine<float, 4, 256>;
+  case 5: return (const void *)&kron_runs_combine<float, 5, 256>;
+  case 6: return (const void *)&kron_runs_combine<float, 6, 256>;
+  default: return nullptr;
+  }
+}
 
 template<>
 bool runs_kernel<double>(int d, bool exact, KernelInfo *out, int *nmax, int *small_max)

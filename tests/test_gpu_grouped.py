@@ -101,6 +101,41 @@ def test_grouped_persistent_long_runs_both_paths(km):
         assert np.array_equal(outs[0][n], outs[1][n])
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_grouped_persistent_split_runs(km, fp32):
+    """One output per size, about 180 entries of n = 7 into one: the one-launch
+    mode cuts the run into parts over several workgroups and a second launch
+    adds the parts in order, so the result is still bitwise equal from call to
+    call; float data with double accumulation as well."""
+    import torch
+    d, sizes = 4, [2, 7]
+    ns, yg, mats, X, Y0 = _make(313, d, sizes, 360, 1)
+    want = _oracle(d, ns, yg, mats, X, Y0)
+    if not fp32:
+        outs, (nout, ngroups, _) = _gpu(km, d, ns, yg, mats, X, Y0, runs=2, mode="persistent")
+        assert nout == 2 and ngroups > 2
+        for n in want:
+            assert rel_err(outs[0][n], want[n]) <= RTOL_FP64, n
+            assert np.array_equal(outs[0][n], outs[1][n])
+        return
+    dev = torch.device("cuda:0")
+    tY = {n: torch.tensor(v, device=dev, dtype=torch.float32) for n, v in Y0.items()}
+    tX = [torch.tensor(x, device=dev, dtype=torch.float32) for x in X]
+    tA = [torch.tensor(np.ascontiguousarray(mats[b][k].T), device=dev, dtype=torch.float32)
+          for b in range(len(ns)) for k in range(d)]
+    table = torch.tensor([t.data_ptr() for t in tA], dtype=torch.int64, device=dev)
+    pX = torch.tensor([t.data_ptr() for t in tX], dtype=torch.int64, device=dev)
+    pY = torch.tensor([tY[n][yg[b]].data_ptr() for b, n in enumerate(ns)], dtype=torch.int64, device=dev)
+    plan = km.GroupPlan(d, list(ns), pY, elem_bytes=4)
+    try:
+        km.kronmult_grouped(plan, table, pX, pY, opt=km.options(persistent=True), dtype="f32")
+        torch.cuda.synchronize()
+    finally:
+        plan.close()
+    for n in want:
+        assert rel_err(tY[n].cpu().numpy().astype(np.float64), want[n]) <= 1e-5, n
+
+
 def test_grouped_large_bins_vs_oracle(km):
     """Bins larger than one resident wave of workgroups, so the even split
     cuts output runs and the atomic flush of a shared run is exercised."""

@@ -133,11 +133,14 @@ def test_xbatched_plan_with_regrouped_outputs(km):
         plan.close()
 
 
+# (4, (3, 8), 2): long runs of n = 8, which the one-launch mode cuts into
+# parts combined by a second launch (plan outputs) or adds with atomics
+@pytest.mark.parametrize("d,sizes,groups", [(3, (2, 4, 5), 6), (4, (3, 8), 2)])
 @pytest.mark.parametrize("mode", ["deterministic", "persistent"])
-def test_grouped_owned_modes_with_regrouped_outputs(km, mode):
+def test_grouped_owned_modes_with_regrouped_outputs(km, mode, d, sizes, groups):
     import torch
     dev = _dev()
-    d, sizes, batch, groups = 3, (2, 4, 5), 240, 6
+    batch = 240
     rng = np.random.default_rng(950)
     ns = rng.choice(sizes, batch)
     g1 = np.zeros(batch, dtype=np.int64)
