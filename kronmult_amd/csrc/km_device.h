@@ -9,8 +9,8 @@
 // Kernel families:
 //   kron_lds   whole n^d tensor of the entry in LDS; every mode contraction is
 //              a pass of per-thread fibers (n loads, n*n FMAs, n stores);
-//   kron_reg   n a power of two: groups of modes contracted in registers, one
-//              LDS transpose between groups;
+//   kron_reg   n^(d-RM) threads of n^RM values each: groups of modes
+//              contracted in registers, one LDS transpose between groups;
 //   kron_slab  n^d too large for LDS but n^(d-1) is not: the input is streamed
 //              in slabs X(j1,:) (direct-to-LDS loads, double buffered), modes
 //              2..d are contracted inside the slab and mode 1 is accumulated
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(NT) void kron_lds(EntryArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------
-// kron_reg: register-blocked passes for n a power of two.  The n^d tensor of
+// kron_reg: register-blocked passes (d6n4, d3n8, d4n4..d4n8).  The n^d tensor of
 // an entry is held by NTH = n^(d-RM) threads with RR = n^RM elements each.
 // Pass g keeps the modes [g*RM, (g+1)*RM) in registers and contracts them
 // there (A_k in SGPRs, RR/n independent fibers per mode); between passes the
@@ -628,7 +628,7 @@ __device__ __forceinline__ void reg_group(typename Acc<T>::type (&v)[cpow(N, RM)
   using C = typename Acc<T>::type;
   constexpr int RR = cpow(N, RM);
   constexpr int ST = cpow(N, RM - 1 - M);
-  constexpr int RB = (N <= 4) ? N : N / 2; // A rows held in SGPRs at a time
+  constexpr int RB = (N <= 4) ? N : (N + 1) / 2; // A rows held in SGPRs at a time
   const T *Ak = a_of(a, b, g * RM + M, D);
   const int lda = a.lda ? a.lda : N;
   if constexpr (RB == N)
@@ -679,7 +679,8 @@ __device__ __forceinline__ void reg_group(typename Acc<T>::type (&v)[cpow(N, RM)
     for (int j = 0; j < N; j++)
 #pragma unroll
       for (int i = 0; i < RB; i++)
-        am[i][j] = static_cast<C>(cld(Ak + (r0 + i + j * lda)));
+        if (r0 + i < N)
+          am[i][j] = static_cast<C>(cld(Ak + (r0 + i + j * lda)));
 #pragma unroll
     for (int base = 0; base < RR; base++)
     {
@@ -688,6 +689,8 @@ __device__ __forceinline__ void reg_group(typename Acc<T>::type (&v)[cpow(N, RM)
 #pragma unroll
       for (int i = 0; i < RB; i++)
       {
+        if (r0 + i >= N)
+          continue; // odd n: the last block has fewer rows
         C acc;
         if constexpr (EXACT)
         {
@@ -743,7 +746,8 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
   constexpr int NTH = S / RR;
   constexpr int NG = D / RM;
   static_assert(D % RM == 0, "mode groups must tile d");
-  static_assert(NTH % 64 == 0, "whole wavefronts");
+  // NTH need not be a whole number of wavefronts (n = 5, 6, 7): the last
+  // wave runs with fewer lanes; every access below is indexed by tid < NTH
   const int tid = threadIdx.x;
 
   C v[RR];

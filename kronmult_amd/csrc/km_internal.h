@@ -103,9 +103,10 @@ hipError_t tables_launch(TableArgs<T> &t, int cus, hipStream_t stream);
 #ifndef KM_USE_REG
 #define KM_USE_REG 1
 #endif
-// kron_reg instantiation for (d, n) if one was built (forward mode order)
+// kron_reg instantiation for (d, n) if one was built (forward mode order);
+// all: also the ones that are not a default (tuned variants, owned bins)
 template<typename T>
-bool reg_kernel(int d, int n, bool exact, KernelInfo *out);
+bool reg_kernel(int d, int n, bool exact, KernelInfo *out, bool all = false);
 
 // kron_slab instantiation for (d, n) if one was built
 template<typename T>

@@ -1,4 +1,9 @@
-// kron_reg instantiations (register-blocked mode groups, n a power of two).
+// kron_reg instantiations (register-blocked mode groups).  d4n6 (216
+// threads) is a default; d4n5 and d4n7 (125 and 343 threads) only serve
+// owned grouped bins and tuned variants: plain calls measured slower there
+// than kron_lds (d4n5 x65536 4:1 0.134 against 0.121 ms, d4n7 x20000 0.214
+// against 0.173 ms), owned bins of 2857 entries faster (d4n5 0.018 against
+// 0.032 ms, d4n7 0.057 against 0.073 ms); d4n6 x65536 0.310 against 0.341.
 #include "km_internal.h"
 
 namespace km
@@ -15,22 +20,34 @@ namespace km
   }
 
 template<>
-bool reg_kernel<double>(int d, int n, bool exact, KernelInfo *out)
+bool reg_kernel<double>(int d, int n, bool exact, KernelInfo *out, bool all)
 {
   KM_REG_CASE(double, 6, 4, 2)
   KM_REG_CASE(double, 3, 8, 1)
   KM_REG_CASE(double, 4, 8, 1)
   KM_REG_CASE(double, 4, 4, 1)
+  KM_REG_CASE(double, 4, 6, 1)
+  if (all)
+  {
+    KM_REG_CASE(double, 4, 5, 1)
+    KM_REG_CASE(double, 4, 7, 1)
+  }
   return false;
 }
 
 template<>
-bool reg_kernel<float>(int d, int n, bool exact, KernelInfo *out)
+bool reg_kernel<float>(int d, int n, bool exact, KernelInfo *out, bool all)
 {
   KM_REG_CASE(float, 6, 4, 2)
   KM_REG_CASE(float, 3, 8, 1)
   KM_REG_CASE(float, 4, 8, 1)
   KM_REG_CASE(float, 4, 4, 1)
+  KM_REG_CASE(float, 4, 6, 1)
+  if (all)
+  {
+    KM_REG_CASE(float, 4, 5, 1)
+    KM_REG_CASE(float, 4, 7, 1)
+  }
   return false;
 }
 

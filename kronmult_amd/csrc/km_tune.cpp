@@ -183,7 +183,7 @@ bool variant_kernel(int v, int d, int n, KernelInfo *k)
   case KM_KERNEL_SLAB: return slab_kernel<T>(d, n, false, k);
   case KM_KERNEL_HALF1: return half_kernel<T>(d, n, 1, k);
   case KM_KERNEL_HALF2: return half_kernel<T>(d, n, 2, k);
-  case KM_KERNEL_REG: return reg_kernel<T>(d, n, false, k);
+  case KM_KERNEL_REG: return reg_kernel<T>(d, n, false, k, true);
   case KM_KERNEL_LDS: return lds_kernel<T>(d, n, false, false, k);
   case KM_KERNEL_LDS_WIDE: return lds_kernel_wide<T>(d, n, k);
   case KM_KERNEL_TWOPASS:
