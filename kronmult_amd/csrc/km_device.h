@@ -830,8 +830,20 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
   flush(target);
 }
 
+// Waves per SIMD the register allocation of kron_reg must allow.  The d4n8
+// FMA kernel takes 87 VGPRs by itself (two 512-thread workgroups per CU);
+// capped at 80 (4 VGPRs spilled) three fit: d4n8 x20000 10:1 atomic 0.272 ->
+// 0.257 ms, plan 0.267 -> 0.241 ms.  Elsewhere the cap changed nothing or
+// spilled more.
+template<typename T, int D, int N, bool EXACT>
+constexpr int reg_waves()
+{
+  return (sizeof(T) == 8 && D == 4 && N == 8 && !EXACT) ? 6 : 1;
+}
+
 template<typename T, int D, int N, int RM, bool EXACT>
-__global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
+__global__ __launch_bounds__(cpow(N, D - RM))
+__attribute__((amdgpu_waves_per_eu(reg_waves<T, D, N, EXACT>()))) void kron_reg(EntryArgs<T> a)
 {
   plan_bind(a);
   __shared__ T buf[cpow(N, D) + cpow(N, D - RM)];
