@@ -194,9 +194,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mode", choices=["best", "atomic", "plan"], default="best",
+    ap.add_argument("--mode", choices=["best", "atomic", "plan", "persistent"], default="best",
                     help="output reduction: atomic adds (drop-in), a deterministic plan, or "
-                         "time both and report the faster")
+                         "time both and report the faster; grouped only: persistent times the "
+                         "one-launch call (KM_FLAG_PERSISTENT) alone")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
@@ -218,6 +219,9 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the batched kernel on one "
               "GPU's share); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    if args.mode == "persistent" and args.workload != "grouped":
+        print("bench.py: --mode persistent applies to --workload grouped only", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -623,7 +627,8 @@ def bench_grouped(args, dev):
 
     # --mode atomic / plan time the default or the deterministic call alone
     # (for profiling); the default mode times all four
-    only = {"atomic": "grouped", "plan": "grouped_deterministic"}.get(args.mode)
+    only = {"atomic": "grouped", "plan": "grouped_deterministic",
+            "persistent": "grouped_persistent"}.get(args.mode)
     results = {name: timed(fn) for name, fn in calls.items() if only in (None, name)}
     ms_per_step, wall = results[only or "grouped"]
 
@@ -670,7 +675,8 @@ def bench_grouped(args, dev):
             "sample": f"first {m} of {batch} entries, one kronmult{d}_pbatched call per size "
                       f"(default recursive build), median of 3 per size, OpenMP threads={O.num_threads()}",
         }
-    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"grouped_d{d}_b{batch}.txt"))
+    kind = "kron_runs_" if only == "grouped_persistent" else ""
+    traffic = parse_traffic(os.path.join(ROOT, "profiles", f"grouped_{kind}d{d}_b{batch}.txt"))
     out = {
         "metric": "d=6 batched kronmult GFLOP/s + % HBM roofline; mixed vs FP64 at rel err <=1e-10",
         "value": round(fl / (ms_per_step * 1e-3) / 1e9, 2),
@@ -688,10 +694,12 @@ def bench_grouped(args, dev):
             "workload": f"kronmult_grouped FP64, d={d}, {batch} entries, n in {list(sizes)}, {groups} output "
                         f"groups per n, each entry into a random group of its n",
             "d": d, "batch": batch,
-            "reduction": ("owned outputs (KM_FLAG_DETERMINISTIC)" if only == "grouped_deterministic"
-                          else "atomic adds (per-size launches, split evenly)"),
+            "reduction": {"grouped_deterministic": "owned outputs (KM_FLAG_DETERMINISTIC)",
+                          "grouped_persistent": "owned outputs, one launch (KM_FLAG_PERSISTENT)"}.get(
+                              only, "atomic adds (per-size launches, split evenly)"),
             "ms_per_step_by_call": {k: round(v[0], 5) for k, v in results.items()},
-            "kernel": "one kron_reg / kron_lds launch per size bin",
+            "kernel": ("kron_runs (one launch; a second short one combines split runs)"
+                       if only == "grouped_persistent" else "one kron_reg / kron_lds launch per size bin"),
             "parallelism": "single GPU",
         },
         "roofline": {
