@@ -221,7 +221,7 @@ int device_cus()
   return v;
 }
 
-int occupancy(const void *fn, int nt)
+int occupancy_fn(const void *fn, int nt)
 {
   static std::map<std::pair<int, const void *>, int> cache;
   int dev = 0;
@@ -240,6 +240,14 @@ int occupancy(const void *fn, int nt)
   std::lock_guard<std::mutex> lk(g_mu);
   cache[{dev, fn}] = nb;
   return nb;
+}
+
+// workgroups per CU a launch of k is sized for: as many as fit, or fewer
+// where the kernel records a measured limit (KernelInfo::max_per_cu)
+int occupancy(const KernelInfo &k)
+{
+  const int nb = occupancy_fn(k.fn, k.nt);
+  return k.max_per_cu > 0 ? std::min(nb, k.max_per_cu) : nb;
 }
 
 // kron_xcd runs on MI355X's layout (gfx950: eight XCDs, the CUs split evenly
@@ -497,8 +505,8 @@ km_status launch_twopass(Call<T> &c, const Resolved &r, const KernelInfo &k1,
     if (bs != KM_SUCCESS)
       return bs;
   }
-  const int occ1 = occupancy(k1.fn, k1.nt);
-  const int occ2 = occupancy(k2.fn, k2.nt);
+  const int occ1 = occupancy(k1);
+  const int occ2 = occupancy(k2);
   const int gy = (int)((M + k2.nt - 1) / k2.nt);
   for (long long e0 = 0; e0 < batch; e0 += chunk_entries)
   {
@@ -623,8 +631,8 @@ km_status launch_twopass_fwd(Call<T> &c, const Resolved &r, const KernelInfo &k1
     if (bs != KM_SUCCESS)
       return bs;
   }
-  const int occ1 = occupancy(k1.fn, k1.nt);
-  const int occ2 = occupancy(k2.fn, k2.nt);
+  const int occ1 = occupancy(k1);
+  const int occ2 = occupancy(k2);
   const int gy1 = (int)((M + k1.nt - 1) / k1.nt);
   for (long long e0 = 0; e0 < batch; e0 += chunk_entries)
   {
@@ -765,7 +773,7 @@ km_status launch(Call<T> &c, const Resolved &r0, hipStream_t stream)
   }
   else
   {
-    long long g = (long long)cus * occupancy(k.fn, k.nt);
+    long long g = (long long)cus * occupancy(k);
     if (halves)
       g = std::max(1LL, g / 2);
     if (!onchip)
@@ -1284,10 +1292,10 @@ long long plan_groups(int d, int n, int elem_bytes, int batch)
   {
     // pass 2 runs one workgroup per (entry group, block of positions)
     const long long gy = (ipow(n, d - 2) + k2.nt - 1) / k2.nt;
-    G = std::max(1LL, (long long)device_cus() * occupancy(k2.fn, k2.nt) * 2 / gy);
+    G = std::max(1LL, (long long)device_cus() * occupancy(k2) * 2 / gy);
   }
   else
-    G = (long long)device_cus() * occupancy(k.fn, k.nt);
+    G = (long long)device_cus() * occupancy(k);
   if (onchip && k.family == km::FAM_HALF)
     G = std::max(1LL, G / 2); // ranges are served by pairs of workgroups
   G = std::min<long long>(G, km::kOrderMaxGroups);
@@ -1794,7 +1802,7 @@ extern "C"
     constexpr double kSplitShares = 2.0;
     constexpr int kSplitParts = 16;
     const long long G = std::max(
-        1LL, std::min<long long>((long long)device_cus() * occupancy(k.fn, k.nt),
+        1LL, std::min<long long>((long long)device_cus() * occupancy(k),
                                  batchCount));
     std::vector<int> ranges;
     ranges.push_back(0);
@@ -1887,7 +1895,7 @@ extern "C"
       const bool have = (elem_bytes == 8) ? bin_kernel<double>(d, n, false, &kb)
                                           : bin_kernel<float>(d, n, false, &kb);
       const long long cnt = j - i;
-      long long g = have ? (long long)device_cus() * occupancy(kb.fn, kb.nt) : 1;
+      long long g = have ? (long long)device_cus() * occupancy(kb) : 1;
       // a bin that fits kWideWaves resident waves takes the wider kron_lds
       // for FMA calls, and its split is sized for it
       km_group_bin bn;
@@ -1898,7 +1906,7 @@ extern "C"
                              : km::lds_kernel_wide<float>(d, n, &kw)))
       {
         bn.wide = 1;
-        g = (long long)device_cus() * occupancy(kw.fn, kw.nt);
+        g = (long long)device_cus() * occupancy(kw);
       }
       g = std::max(1LL, std::min(g, cnt));
       bn.n = n;

@@ -736,6 +736,43 @@ __device__ __forceinline__ int reg_pad(int idx)
   return idx + idx / RR;
 }
 
+// L_g of reg_index as a compile-time value
+template<int D, int N, int RM>
+constexpr int reg_l(int g)
+{
+  int L = 1;
+  for (int k = 0; k < D - (g + 1) * RM; k++)
+    L *= N;
+  return L;
+}
+
+// Padded position of register r of thread t in layout g: the position of
+// register 0 plus r * reg_step(g).  With L_g a multiple of RR the pad term
+// idx / RR grows by L_g / RR per register; with L_g = 1 the registers are
+// idx = t*RR + r, r < RR, and the pad term stays t.  So each transpose needs
+// one address per thread and compile-time offsets, instead of one address
+// per register (which took VGPRs enough to spill at d4n8).
+template<int D, int N, int RM>
+constexpr int reg_step(int g)
+{
+  constexpr int RR = cpow(N, RM);
+  const int L = reg_l<D, N, RM>(g);
+  return L == 1 ? 1 : L + L / RR;
+}
+
+template<int D, int N, int RM>
+constexpr bool reg_steps_exact()
+{
+  constexpr int RR = cpow(N, RM);
+  for (int g = 0; g < D / RM; g++)
+  {
+    const int L = reg_l<D, N, RM>(g);
+    if (L != 1 && L % RR != 0)
+      return false;
+  }
+  return true;
+}
+
 template<typename T, int D, int N, int RM, bool EXACT, bool PRELOAD, bool XNT>
 __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
                                          int p1)
@@ -746,6 +783,8 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
   constexpr int NTH = S / RR;
   constexpr int NG = D / RM;
   static_assert(D % RM == 0, "mode groups must tile d");
+  static_assert(reg_steps_exact<D, N, RM>() && NTH % RR == 0,
+                "transpose positions linear in the register index");
   // NTH need not be a whole number of wavefronts (n = 5, 6, 7): the last
   // wave runs with fewer lanes; every access below is indexed by tid < NTH
   const int tid = threadIdx.x;
@@ -768,14 +807,21 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
       return;
     }
     lds_barrier();
+    {
+      T *w = buf + reg_pad<RR>(reg_index<D, N, RM>(NG - 1, tid, 0));
 #pragma unroll
-    for (int r = 0; r < RR; r++)
-      buf[reg_pad<RR>(reg_index<D, N, RM>(NG - 1, tid, r))] = acc[r];
+      for (int r = 0; r < RR; r++)
+        w[r * reg_step<D, N, RM>(NG - 1)] = acc[r];
+    }
     lds_barrier();
     T out[RR];
+    {
+      // tid + NTH*k: NTH is a multiple of RR
+      const T *q = buf + reg_pad<RR>(tid);
 #pragma unroll
-    for (int k = 0; k < RR; k++)
-      out[k] = buf[reg_pad<RR>(tid + NTH * k)];
+      for (int k = 0; k < RR; k++)
+        out[k] = q[k * (NTH + NTH / RR)];
+    }
     flush_run<RR, NTH, S, RR>(dst, out, tid, a.owned);
   };
 
@@ -813,13 +859,15 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
       if (g + 1 < NG)
       {
         lds_barrier(); // previous readers of buf are done
+        T *w = buf + reg_pad<RR>(reg_index<D, N, RM>(g, tid, 0));
 #pragma unroll
         for (int r = 0; r < RR; r++)
-          buf[reg_pad<RR>(reg_index<D, N, RM>(g, tid, r))] = static_cast<T>(v[r]);
+          w[r * reg_step<D, N, RM>(g)] = static_cast<T>(v[r]);
         lds_barrier();
+        const T *q = buf + reg_pad<RR>(reg_index<D, N, RM>(g + 1, tid, 0));
 #pragma unroll
         for (int r = 0; r < RR; r++)
-          v[r] = static_cast<C>(buf[reg_pad<RR>(reg_index<D, N, RM>(g + 1, tid, r))]);
+          v[r] = static_cast<C>(q[r * reg_step<D, N, RM>(g + 1)]);
       }
     }
 #pragma unroll
@@ -830,20 +878,8 @@ __device__ __forceinline__ void reg_body(const EntryArgs<T> &a, T *buf, int p0,
   flush(target);
 }
 
-// Waves per SIMD the register allocation of kron_reg must allow.  The d4n8
-// FMA kernel takes 87 VGPRs by itself (two 512-thread workgroups per CU);
-// capped at 80 (4 VGPRs spilled) three fit: d4n8 x20000 10:1 atomic 0.272 ->
-// 0.257 ms, plan 0.267 -> 0.241 ms.  Elsewhere the cap changed nothing or
-// spilled more.
-template<typename T, int D, int N, bool EXACT>
-constexpr int reg_waves()
-{
-  return (sizeof(T) == 8 && D == 4 && N == 8 && !EXACT) ? 6 : 1;
-}
-
 template<typename T, int D, int N, int RM, bool EXACT>
-__global__ __launch_bounds__(cpow(N, D - RM))
-__attribute__((amdgpu_waves_per_eu(reg_waves<T, D, N, EXACT>()))) void kron_reg(EntryArgs<T> a)
+__global__ __launch_bounds__(cpow(N, D - RM)) void kron_reg(EntryArgs<T> a)
 {
   plan_bind(a);
   __shared__ T buf[cpow(N, D) + cpow(N, D - RM)];

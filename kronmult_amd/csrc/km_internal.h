@@ -35,6 +35,8 @@ struct KernelInfo
   int nt = 0;               // threads per workgroup
   int family = FAM_LDS;
   int per_cu = 1;           // workgroups per CU of a cooperative launch
+  int max_per_cu = 0;       // resident workgroups per CU the grid is sized
+                            // for (0: as many as fit)
 };
 
 // threads per workgroup of kron_lds for a tensor of s elements

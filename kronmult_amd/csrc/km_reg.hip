@@ -9,28 +9,33 @@
 namespace km
 {
 
-#define KM_REG_CASE(T, D, N, RM)                                               \
+// CAP: resident workgroups per CU the grid is sized for (0: all that fit).
+// d3n8 (61 VGPRs, 64 threads: 32 fit) measured best at 24: x262144 4:1
+// 0.461 ms at 24 (where its registers held it before the transposes used
+// one address per thread), 0.523 ms at 32.
+#define KM_REG_CASE(T, D, N, RM, CAP)                                          \
   if (d == D && n == N)                                                        \
   {                                                                            \
     out->fn = exact ? (const void *)&kron_reg<T, D, N, RM, true>               \
                     : (const void *)&kron_reg<T, D, N, RM, false>;             \
     out->nt = cpow(N, D - RM);                                                 \
     out->family = FAM_REG;                                                     \
+    out->max_per_cu = CAP;                                                     \
     return true;                                                               \
   }
 
 template<>
 bool reg_kernel<double>(int d, int n, bool exact, KernelInfo *out, bool all)
 {
-  KM_REG_CASE(double, 6, 4, 2)
-  KM_REG_CASE(double, 3, 8, 1)
-  KM_REG_CASE(double, 4, 8, 1)
-  KM_REG_CASE(double, 4, 4, 1)
-  KM_REG_CASE(double, 4, 6, 1)
+  KM_REG_CASE(double, 6, 4, 2, 0)
+  KM_REG_CASE(double, 3, 8, 1, 24)
+  KM_REG_CASE(double, 4, 8, 1, 0)
+  KM_REG_CASE(double, 4, 4, 1, 0)
+  KM_REG_CASE(double, 4, 6, 1, 0)
   if (all)
   {
-    KM_REG_CASE(double, 4, 5, 1)
-    KM_REG_CASE(double, 4, 7, 1)
+    KM_REG_CASE(double, 4, 5, 1, 0)
+    KM_REG_CASE(double, 4, 7, 1, 0)
   }
   return false;
 }
@@ -38,15 +43,15 @@ bool reg_kernel<double>(int d, int n, bool exact, KernelInfo *out, bool all)
 template<>
 bool reg_kernel<float>(int d, int n, bool exact, KernelInfo *out, bool all)
 {
-  KM_REG_CASE(float, 6, 4, 2)
-  KM_REG_CASE(float, 3, 8, 1)
-  KM_REG_CASE(float, 4, 8, 1)
-  KM_REG_CASE(float, 4, 4, 1)
-  KM_REG_CASE(float, 4, 6, 1)
+  KM_REG_CASE(float, 6, 4, 2, 0)
+  KM_REG_CASE(float, 3, 8, 1, 24)
+  KM_REG_CASE(float, 4, 8, 1, 0)
+  KM_REG_CASE(float, 4, 4, 1, 0)
+  KM_REG_CASE(float, 4, 6, 1, 0)
   if (all)
   {
-    KM_REG_CASE(float, 4, 5, 1)
-    KM_REG_CASE(float, 4, 7, 1)
+    KM_REG_CASE(float, 4, 5, 1, 0)
+    KM_REG_CASE(float, 4, 7, 1, 0)
   }
   return false;
 }
