@@ -75,7 +75,9 @@ def test_tune_two_pass_shape(km):
 
 
 VARIANTS = [(6, 6, "HALF1"), (6, 6, "HALF2"), (5, 8, "HALF1"), (5, 8, "HALF2"), (6, 6, "SLAB"),
-            (6, 4, "LDS"), (4, 5, "LDS_WIDE"), (3, 8, "LDS"), (4, 4, "REG")]
+            (6, 4, "LDS"), (4, 5, "LDS_WIDE"), (3, 8, "LDS"), (4, 4, "REG"),
+            # kron_reg with a partly filled last wave (125 and 343 threads)
+            (4, 5, "REG"), (4, 7, "REG")]
 
 
 @pytest.mark.parametrize("d,n,name", VARIANTS)
