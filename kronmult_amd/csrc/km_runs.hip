@@ -28,14 +28,35 @@
 namespace km
 {
 
-template<typename T, int D, int NMAX, int NT>
+// Whether large entries take the register-blocked body (runs_reg_unit): FMA
+// calls at d = 4 with 512 threads, where the large sizes n = 6, 7, 8 have
+// n^3 = 216, 343, 512 <= 512 threads of n values each.
+template<int D, int NT, bool EXACT>
+constexpr bool runs_reg_geom()
+{
+  return !EXACT && D == 4 && NT == 512;
+}
+
+constexpr int cmax(int x, int y) { return x > y ? x : y; }
+
+// REG: entries of the largest size n = NMAX take the register-blocked body,
+// whose LDS is one padded transpose buffer (n^d + n^(d-1)); the other large
+// ones are double-buffered in LDS as below, in two halves of the same
+// buffer; small entries get 636-element regions (n <= 5 at d = 4): 40 KiB in
+// all.  Otherwise every large entry is double-buffered in LDS (2 n^d).
+template<typename T, int D, int NMAX, int NT, bool REG = false>
 struct RunsGeom
 {
   static constexpr int SMAX = cpow(NMAX, D);               // largest entry
   static constexpr int NW = NT / 64;                        // wavefronts
-  static constexpr int LDSE = 2 * SMAX;                     // LDS elements
-  static constexpr int SW = NW > 1 ? (LDSE / NW) & ~3 : 0;  // per-wave region
-  static constexpr int RB = (SMAX + NT - 1) / NT;           // large-entry sum per thread
+  static constexpr int SWR = 636;                           // REG: per-wave region
+  static constexpr int LDSE = REG ? cmax(SMAX + cpow(NMAX, D - 1), NW * SWR)
+                                  : 2 * SMAX;               // LDS elements
+  static constexpr int SW = REG ? SWR : (NW > 1 ? (LDSE / NW) & ~3 : 0);  // per-wave region
+  static constexpr int DMAX = REG ? cpow(NMAX - 1, D) : SMAX; // largest double-buffered entry
+  static constexpr int BSTR = REG ? (LDSE / 2) & ~1 : SMAX; // offset of the second buffer
+  static_assert(DMAX <= BSTR, "two entries fit");
+  static constexpr int RB = (DMAX + NT - 1) / NT;           // large-entry sum per thread
   static constexpr int RS = NW > 1 ? (SW + 63) / 64 : 1;    // small-entry sum per lane
 };
 
@@ -182,14 +203,15 @@ template<typename T, int D, int NMAX, int NT, bool EXACT, bool LARGE, int R>
 __device__ __forceinline__ void runs_dispatch(int n, T *buf, const EntryArgs<T> &a, int b,
                                               int tid, T (&acc)[R])
 {
-  using G = RunsGeom<T, D, NMAX, NT>;
+  using G = RunsGeom<T, D, NMAX, NT, runs_reg_geom<D, NT, EXACT>()>;
   constexpr int NTH = LARGE ? NT : 64;
   switch (n)
   {
 #define KM_RUNS_N(NN)                                                          \
   case NN:                                                                     \
-    if constexpr (NN <= NMAX && (LARGE ? (G::NW == 1 || cpow(NN, D) > G::SW)   \
-                                       : (G::NW > 1 && cpow(NN, D) <= G::SW))) \
+    if constexpr (NN <= NMAX && (LARGE ? ((G::NW == 1 || cpow(NN, D) > G::SW) && \
+                                          cpow(NN, D) <= G::DMAX)                 \
+                                       : (G::NW > 1 && cpow(NN, D) <= G::SW)))    \
       runs_entry<T, D, NN, NTH, R, EXACT, !LARGE>(buf, a, b, tid, acc);        \
     break;
     KM_RUNS_N(1)
@@ -245,10 +267,152 @@ __global__ __launch_bounds__(NT) void kron_runs_combine(EntryArgs<T> a, RunsArgs
   }
 }
 
-template<typename T, int D, int NMAX, int NT, bool EXACT>
-__global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, RunsArgs<T> ra)
+
+// One work unit [ustart[cu], ustart[cu+1]) of size-N entries with the
+// register-blocked body: thread t < N^(D-1) loads X(r*N^(D-1) + t), contracts
+// modes 1..d in registers with one LDS transpose between modes (buf, padded
+// as in kron_reg), and sums the unit's entries in registers in the last
+// layout, where it holds elements t*N .. t*N + N-1.  The other threads
+// contract zeros and read and write a spare LDS slot instead: inside the
+// entry loop no branch depends on the thread, so the factors stay in scalar
+// registers (a lane-divergent branch there made the compiler load them per
+// lane, 61 -> 180 VGPRs).  Forward mode order, as kron_reg (FMA calls only:
+// the order changes rounding alone).  OWNED: the plan's outputs (a plain
+// read-modify-write, or a part sum of a split run, at the end of the unit);
+// otherwise every change of output is flushed with atomics.
+template<typename T, int D, int N, int NT, bool OWNED>
+__device__ __forceinline__ void runs_reg_unit(const EntryArgs<T> &a, const RunsArgs<T> &ra,
+                                              int cu, T *buf, int tid_in)
 {
-  using G = RunsGeom<T, D, NMAX, NT>;
+  // tid through an empty asm statement: the per-thread offsets below are
+  // computed per unit, not hoisted out of the caller's loop over units
+  // (where the offsets of all three sizes stayed live: 116 VGPRs)
+  int tid = tid_in;
+  asm volatile("" : "+v"(tid));
+  using C = typename Acc<T>::type;
+  constexpr int RM = 1;
+  constexpr int RR = N;
+  constexpr int S = cpow(N, D);
+  constexpr int NTH = cpow(N, D - 1);
+  constexpr int NG = D;
+  // padded transpose positions end below S + NTH; the spare slot follows
+  constexpr int SPARE = S + NTH;
+  static_assert(NTH <= NT && SPARE + 1 <= RunsGeom<T, D, 8, NT, true>::LDSE,
+                "the unit fits the workgroup and its LDS");
+  const bool act = (NTH == NT) || tid < NTH;
+  const int t = act ? tid : 0; // idle threads read X(r*NTH) (discarded)
+  const int p0 = cld(ra.ustart + cu), p1 = cld(ra.ustart + cu + 1);
+  T acc[RR];
+#pragma unroll
+  for (int r = 0; r < RR; r++)
+    acc[r] = 0;
+  T *target = y_of(a, entry_at(a, p0));
+  for (int p = p0; p < p1; ++p)
+  {
+    const int b = entry_at(a, p);
+    if constexpr (!OWNED)
+    {
+      // the caller's outputs differ from the plan's: flush where they change
+      T *yb = y_of(a, b);
+      if (yb != target)
+      {
+        if (act)
+        {
+#pragma unroll
+          for (int r = 0; r < RR; r++)
+            gatomic_add(target + tid * RR + r, acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < RR; r++)
+          acc[r] = 0;
+        target = yb;
+      }
+    }
+    C v[RR];
+    {
+      const T *xp = x_of(a, b);
+#pragma unroll
+      for (int r = 0; r < RR; r++)
+        v[r] = static_cast<C>(gld(xp + r * NTH + t));
+    }
+    T pf = 0;
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+    {
+      if (g == 0 && kPrefetchA<N> && p + 1 < p1)
+        pf = prefetch_a<D, N>(a, entry_at(a, p + 1), tid);
+      reg_group<T, D, N, RM, false, 0>(v, a, b, g);
+      if (g + 1 < NG)
+      {
+        lds_barrier(); // previous readers of buf are done
+        {
+          T *w = buf + (act ? reg_pad<RR>(reg_index<D, N, RM>(g, tid, 0)) : SPARE);
+          const int ST = reg_step<D, N, RM>(g);
+#pragma unroll
+          for (int r = 0; r < RR; r++)
+            w[act ? r * ST : 0] = static_cast<T>(v[r]);
+        }
+        lds_barrier();
+        {
+          const T *q = buf + (act ? reg_pad<RR>(reg_index<D, N, RM>(g + 1, tid, 0)) : SPARE);
+          const int ST = reg_step<D, N, RM>(g + 1);
+#pragma unroll
+          for (int r = 0; r < RR; r++)
+            v[r] = static_cast<C>(q[act ? r * ST : 0]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      acc[r] = acc[r] + static_cast<T>(v[r]);
+    keep_alive(pf);
+  }
+  if (!act)
+    return;
+  if constexpr (!OWNED)
+  {
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      gatomic_add(target + tid * RR + r, acc[r]);
+    return;
+  }
+  const int sid = cld(ra.umeta + 3 * cu);
+  if (sid < 0)
+  {
+    // owned output: read-modify-write of this thread's N elements
+    T yv[RR];
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      yv[r] = gld(target + tid * RR + r);
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      gst(target + tid * RR + r, yv[r] + acc[r]);
+  }
+  else
+  {
+    // one part of a split run: its sum to the plan's workspace
+    T *mine = ra.ws + cld(ra.wsoff + sid) + (long long)cld(ra.umeta + 3 * cu + 1) * S;
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      gst(mine + tid * RR + r, acc[r]);
+  }
+}
+
+// With the register-blocked large entries (FP64), the kernel is compiled for
+// three workgroups per CU (80 VGPRs, 2 of them spilled), which its 40 KiB of
+// LDS allows; unconstrained it took 96 VGPRs (two workgroups per CU).
+template<typename T, int D, int NT, bool EXACT>
+constexpr int runs_waves()
+{
+  return (runs_reg_geom<D, NT, EXACT>() && sizeof(T) == 8) ? 6 : 1;
+}
+
+template<typename T, int D, int NMAX, int NT, bool EXACT>
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(runs_waves<T, D, NT, EXACT>()))) void kron_runs(EntryArgs<T> a, RunsArgs<T> ra)
+{
+  constexpr bool REG = runs_reg_geom<D, NT, EXACT>();
+  using G = RunsGeom<T, D, NMAX, NT, REG>;
   plan_bind(a);
   __shared__ __attribute__((aligned(16))) T lds[G::LDSE];
   __shared__ int scratch[64];
@@ -265,9 +429,30 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, RunsArgs<T> ra)
   while (ub < u1 && !small(cld(nvec + entry_at(a, cld(ra.ustart + ub)))))
     ++ub;
 
-  if (ub > u0)
+  // FMA calls at d = 4: the units of the largest size (n = 8, first in
+  // the plan's order, where sizes descend) with the register-blocked body,
+  // one unit at a time
+  int ur = u0;
+  if constexpr (REG)
   {
-    const int q0 = cld(ra.ustart + u0), qb = cld(ra.ustart + ub);
+    auto unit_n = [&](int u) { return cld(nvec + entry_at(a, cld(ra.ustart + u))); };
+    if (a.owned)
+    {
+      for (; ur < ub && unit_n(ur) == NMAX; ++ur)
+        runs_reg_unit<T, D, NMAX, NT, true>(a, ra, ur, lds, tid);
+    }
+    else
+    {
+      for (; ur < ub && unit_n(ur) == NMAX; ++ur)
+        runs_reg_unit<T, D, NMAX, NT, false>(a, ra, ur, lds, tid);
+    }
+    if (ur > u0 && ur < ub)
+      lds_barrier(); // the last transposes are read before the buffers are reused
+  }
+
+  if (ub > ur)
+  {
+    const int q0 = cld(ra.ustart + ur), qb = cld(ra.ustart + ub);
     T acc[G::RB];
 #pragma unroll
     for (int r = 0; r < G::RB; r++)
@@ -278,7 +463,7 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, RunsArgs<T> ra)
     // output run) being summed
     T *target = y_of(a, b);
     int tn = n, tsize = ipow_dev(n, D);
-    int cu = u0, ce = cld(ra.ustart + u0 + 1);
+    int cu = ur, ce = cld(ra.ustart + ur + 1);
     auto flush = [&]() {
       if (!a.owned)
       {
@@ -308,8 +493,8 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, RunsArgs<T> ra)
       // offsets, not a select of two pointers: a select lets the compiler
       // turn every LDS read into reads of both buffers and a select
       const int par = (p - q0) & 1;
-      T *cur = lds + par * G::SMAX;
-      T *nxt = lds + (par ^ 1) * G::SMAX;
+      T *cur = lds + par * G::BSTR;
+      T *nxt = lds + (par ^ 1) * G::BSTR;
       if (p > q0)
       {
         T *yb = y_of(a, b);
@@ -444,7 +629,8 @@ __global__ __launch_bounds__(NT) void kron_runs(EntryArgs<T> a, RunsArgs<T> ra)
     out->nt = NT;                                                              \
     out->family = FAM_GROUPED;                                                 \
     *nmax = NMAX;                                                              \
-    *small_max = RunsGeom<T, D, NMAX, NT>::SW;                                 \
+    *small_max = exact ? RunsGeom<T, D, NMAX, NT>::SW                          \
+                       : RunsGeom<T, D, NMAX, NT, runs_reg_geom<D, NT, false>()>::SW; \
     return true;                                                               \
   }
 
