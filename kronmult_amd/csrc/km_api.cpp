@@ -1786,12 +1786,19 @@ extern "C"
     // run side by side on the workgroup's wavefronts but each waits on its
     // own loads and factor reads: (39,300 + 7 c) / waves, fitted to measured
     // d = 4 runs (n = 2 and n = 5 cost 0.39 and 1.1 us per entry where n = 8
-    // costs 7.5).  Every run adds the fixed cost of its flush.
+    // costs 7.5).  Every run adds the fixed cost of its flush.  At d = 4 the
+    // n = 8 entries take the register-blocked body (FMA calls) and weigh
+    // 0.5 c, the double-buffered n = 6, 7 ones 1.2 c: swept on the 2..8 mix,
+    // 0.202 ms with 1 and 1, 0.177 with 0.7 and 1, 0.151 with 0.5 and 1.2
+    // (0.157-0.174 at the neighbouring points).
     const int waves = std::max(1, k.nt / 64);
     auto cost = [&](int b) {
       const double sz = std::pow((double)n_host[b], d);
       const double c = elem_bytes * sz + 0.5 * d * sz * n_host[b];
-      return (waves > 1 && sz <= small_max) ? (39300.0 + 7.0 * c) / waves : c + 2048.0;
+      if (waves > 1 && sz <= small_max)
+        return (39300.0 + 7.0 * c) / waves;
+      const double w = d != 4 ? 1.0 : (n_host[b] == 8 ? 0.5 : 1.2);
+      return w * c + 2048.0;
     };
     constexpr double kRunCost = 25000.0;
     // A run of large entries costing more than kSplitShares workgroup shares
