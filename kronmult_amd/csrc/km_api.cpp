@@ -77,6 +77,12 @@ struct km_group_plan_s
   int *splitfirst = nullptr; // device, per split run: first order position
   int *splitparts = nullptr; // device, per split run: parts
   int nsplit = 0;
+  // the units of small entries alone (one wavefront per entry in kron_runs),
+  // cut into workgroup ranges: owned FMA calls of the per-size path run them
+  // in one kron_runs launch instead of their size bins
+  int *sranges = nullptr; // device, nsgroups + 1 entries (unit indices)
+  int nsgroups = 0;
+  int small_max = 0;      // largest n^d of a small entry (0: none)
   // per-size-bin launch: bin b uses binranges[first .. first+count]
   std::vector<km_group_bin> bins;
   int *binranges = nullptr; // device
@@ -1057,12 +1063,26 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
       xtotal += (long long)bn.cnt * ipow(bn.n, plan->d);
     a.x_stream = x_streams(1, xtotal, sizeof(T));
     const bool owned = (o.flags & KM_FLAG_DETERMINISTIC) != 0;
+    // owned FMA calls: the size bins of small entries (one wavefront per
+    // entry in kron_runs) as one kron_runs launch over the plan's ranges of
+    // their output runs, beside the larger bins.  Measured on the d = 4,
+    // n = 2..8 mix: 0.226 -> 0.186 ms; with n = 2..5 alone 0.179 -> 0.044 ms.
+    KernelInfo kr;
+    int rn = 0, rsm = 0;
+    const bool runs_small = owned && !exact && plan->nsgroups > 0 &&
+                            km::runs_kernel<T>(plan->d, false, &kr, &rn, &rsm) &&
+                            plan->nmax <= rn && rsm == plan->small_max;
+    auto small_bin = [&](const km_group_bin &bn) {
+      return runs_small && ipow(bn.n, plan->d) <= plan->small_max;
+    };
     // resolve every bin's kernel before the fork, so an unsupported size
     // returns with nothing launched and no stream left forked
     std::vector<KernelInfo> kbs(plan->bins.size());
     for (size_t i = 0; i < plan->bins.size(); i++)
     {
       const km_group_bin &bn = plan->bins[i];
+      if (small_bin(bn))
+        continue;
       // owned FMA bins take every kron_reg built (a bin of whole output runs
       // has few, long workgroups: d4n5 and d4n7 bins of 2857 entries measured
       // 0.018 and 0.057 ms against 0.032 and 0.073 with kron_lds)
@@ -1076,21 +1096,30 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     // below a full chip, so the bins run concurrently on the plan's streams.
     // The even split of the atomic path already sizes each bin's grid to
     // fill the chip, and there the fork and join only add latency.
-    const int ns = owned ? (int)plan->streams.size() : 0;
+    // Only as many streams as launches are forked (none for one launch: a
+    // lone kron_runs launch on four forked streams took 0.104 ms where it
+    // takes 0.044 on the caller's stream).
+    int nlaunch = runs_small ? 1 : 0;
+    for (const km_group_bin &bn : plan->bins)
+      nlaunch += small_bin(bn) ? 0 : 1;
+    const int ns = owned ? std::min<int>((int)plan->streams.size(), nlaunch) : 0;
     std::unique_lock<std::mutex> fork_lock(plan->fork_mu, std::defer_lock);
     if (ns > 1)
     {
       fork_lock.lock();
       KM_HIP(hipEventRecord(plan->fork, (hipStream_t)stream));
-      for (hipStream_t st : plan->streams)
-        KM_HIP(hipStreamWaitEvent(st, plan->fork, 0));
+      for (int i = 0; i < ns; i++)
+        KM_HIP(hipStreamWaitEvent(plan->streams[i], plan->fork, 0));
     }
     hipError_t launched = hipSuccess;
+    int nl = 0; // launches so far (dealt out to the streams in turn)
     for (size_t bi = 0; bi < plan->bins.size(); bi++)
     {
       const km_group_bin &bn = plan->bins[bi];
+      if (small_bin(bn))
+        continue;
       const KernelInfo &kb = kbs[bi];
-      hipStream_t bs = ns > 1 ? plan->streams[bi % ns] : (hipStream_t)stream;
+      hipStream_t bs = ns > 1 ? plan->streams[nl++ % ns] : (hipStream_t)stream;
       int grid;
       if (owned)
       {
@@ -1117,6 +1146,26 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
       launched = hipLaunchKernel(kb.fn, dim3(grid), dim3(kb.nt), args, 0, bs);
       if (launched != hipSuccess)
         break;
+    }
+    if (runs_small && launched == hipSuccess)
+    {
+      EntryArgs<T> ar = a;
+      ar.batch = plan->batch;
+      ar.order = plan->order;
+      ar.ranges = plan->sranges;
+      ar.chunk = 0;
+      ar.owned = 1;
+      km::RunsArgs<T> ra;
+      ra.nvec = plan->nvec;
+      ra.ustart = plan->runstart;
+      ra.umeta = plan->unitmeta;
+      ra.ws = static_cast<T *>(plan->splitws);
+      ra.wsoff = plan->splitoff;
+      ra.sfirst = plan->splitfirst;
+      ra.sparts = plan->splitparts;
+      void *args[] = {&ar, &ra};
+      hipStream_t rs = ns > 1 ? plan->streams[nl++ % ns] : (hipStream_t)stream;
+      launched = hipLaunchKernel(kr.fn, dim3(plan->nsgroups), dim3(kr.nt), args, 0, rs);
     }
     // the join also follows a failed launch, so that the caller's stream
     // (or a graph being captured from it) is never left forked
@@ -1820,6 +1869,7 @@ extern "C"
     // separately; a second launch adds their sums to the output in part
     // order
     std::vector<int> starts;    // unit starts (order positions)
+    std::vector<int> sranges;   // ranges over the small entries' units
     std::vector<int> unitmeta;  // split id, part, parts
     std::vector<long long> splitoff;
     std::vector<int> splitfirst, splitparts;
@@ -1882,6 +1932,26 @@ extern "C"
       starts.push_back(batchCount);
       pre.push_back(acc);
       balanced_cuts(starts, pre, G, ranges);
+      // the units of small entries (a suffix: sizes descend) on their own
+      size_t us0 = 0;
+      while (us0 + 1 < starts.size())
+      {
+        const double sz = std::pow((double)n_host[order[starts[us0]]], d);
+        if (waves > 1 && sz <= small_max)
+          break;
+        ++us0;
+      }
+      if (us0 + 1 < starts.size())
+      {
+        std::vector<int> ss(starts.begin() + us0, starts.end());
+        std::vector<double> sp(pre.begin() + us0, pre.end());
+        for (double &v : sp)
+          v -= pre[us0];
+        sranges.push_back(ss[0]);
+        balanced_cuts(ss, sp, G, sranges);
+        for (int &c : sranges)
+          c = (int)(std::lower_bound(starts.begin(), starts.end(), c) - starts.begin());
+      }
       // the cuts fall on unit starts: express them in units
       for (int &c : ranges)
         c = (int)(std::lower_bound(starts.begin(), starts.end(), c) - starts.begin());
@@ -1954,6 +2024,14 @@ extern "C"
       KM_HIP(hipMalloc(&p->unitmeta, sizeof(int) * unitmeta.size()));
       KM_HIP(hipMemcpy(p->unitmeta, unitmeta.data(), sizeof(int) * unitmeta.size(),
                        hipMemcpyHostToDevice));
+      if (sranges.size() > 1)
+      {
+        KM_HIP(hipMalloc(&p->sranges, sizeof(int) * sranges.size()));
+        KM_HIP(hipMemcpy(p->sranges, sranges.data(), sizeof(int) * sranges.size(),
+                         hipMemcpyHostToDevice));
+        p->nsgroups = (int)sranges.size() - 1;
+        p->small_max = waves > 1 ? small_max : 0;
+      }
       if (!splitoff.empty())
       {
         KM_HIP(hipMalloc(&p->splitws, (size_t)elem_bytes * (size_t)split_elems));
@@ -2022,6 +2100,8 @@ extern "C"
       (void)hipFree(plan->runstart);
     if (plan->unitmeta)
       (void)hipFree(plan->unitmeta);
+    if (plan->sranges)
+      (void)hipFree(plan->sranges);
     if (plan->splitws)
       (void)hipFree(plan->splitws);
     if (plan->splitoff)
