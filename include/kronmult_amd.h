@@ -267,8 +267,9 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
  * instead, so every output is owned by one workgroup (no atomics, bitwise
  * reproducible); its bins run concurrently on up to four streams the plan
  * owns, forked from `stream` and joined back into it with events (also
- * capturable); calls from several host threads on one plan take turns
- * through the fork (a per-plan lock).  KM_FLAG_PERSISTENT runs all bins in
+ * capturable; without KM_FLAG_EXACT the small sizes go together into one
+ * launch of the one-launch kernel below); calls from several host threads
+ * on one plan take turns through the fork (a per-plan lock).  KM_FLAG_PERSISTENT runs all bins in
  * one launch with owned outputs (each workgroup takes whole output runs;
  * bitwise reproducible; the fastest owned mode on mixed sizes, within about
  * 1.1x of the default on a d = 4 mix of n = 2..8).  An output run of large entries too long for one workgroup is
