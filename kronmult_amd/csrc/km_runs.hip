@@ -4,25 +4,32 @@
 // grouped by output), so every output is updated by one workgroup with a
 // plain read-modify-write, bitwise reproducible from call to call.
 //
-// Two ways through a run, chosen by the entry size S = n^d:
+// Ways through a run, chosen by the entry size S = n^d:
 //   large (S above one wavefront's LDS region): the whole workgroup
-//       contracts one entry at a time in LDS (one fiber per thread and mode),
-//       the run's sum kept in registers.  LDS holds two entries: while one is
-//       contracted, the next entry of the workgroup's range is copied in by
-//       direct-to-LDS loads and its factors are pulled into L2, so neither
-//       waits on global memory and no registers hold them.
+//       contracts one entry at a time, the run's sum kept in registers.
+//       - double-buffered: contracted in LDS (one fiber per thread and
+//         mode); LDS holds two entries: while one is contracted, the next
+//         entry of the workgroup's range is copied in by direct-to-LDS loads
+//         and its factors are pulled into L2, so neither waits on global
+//         memory and no registers hold them.
+//       - register-blocked (FMA calls at d = 4, the largest size n = 8):
+//         kron_reg's mode groups, X loaded straight into registers, one LDS
+//         transpose between modes (runs_reg_unit).
 //   small: the run's entries are dealt out to the wavefronts (entry k of the
 //       run to wave k mod waves); each wave loads its entry into its own LDS
 //       region and contracts it with wave-level synchronisation only, summing
 //       its entries in registers, while its next entry's X and factors are
 //       pulled into L2.  At the end of the run the waves' partial sums meet
 //       in LDS and are added to Y in a fixed order.
-// A mode pass loads every fiber of a thread first and applies the factor a
-// few rows at a time from scalar registers (stage_apply), so n = 7, 8 need no
-// more SGPRs than one block of rows.  The entries of a run and the waves'
-// partial sums are added in a fixed order, so the result does not depend on
-// timing.  Mode order: d..1, the Alg. 993 backward order
-// (kronm_backward.hpp:92-139), as kron_lds.
+// An LDS mode pass loads every fiber of a thread first and applies the
+// factor a few rows at a time from scalar registers (stage_apply), so n = 7,
+// 8 need no more SGPRs than one block of rows.  The entries of a run and the
+// waves' partial sums are added in a fixed order, so the result does not
+// depend on timing.  Mode order: d..1, the Alg. 993 backward order
+// (kronm_backward.hpp:92-139), as kron_lds; the register-blocked body runs
+// 1..d as kron_reg (FMA calls only, where the order changes rounding alone).
+// The kernel also serves the small sizes of owned per-size grouped calls
+// (the plan's ranges over those runs alone).
 #include "km_internal.h"
 
 namespace km
