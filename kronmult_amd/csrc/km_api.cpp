@@ -1067,6 +1067,9 @@ km_status grouped(km_group_plan plan, const T *const *Aarray, T *const *pX,
     // entry in kron_runs) as one kron_runs launch over the plan's ranges of
     // their output runs, beside the larger bins.  Measured on the d = 4,
     // n = 2..8 mix: 0.226 -> 0.186 ms; with n = 2..5 alone 0.179 -> 0.044 ms.
+    // The default (atomic) calls keep their bins: there one launch per run
+    // caps the parallelism at the number of outputs (n = 2 alone with 285
+    // outputs: 9 -> 33 us), and the 2..8 mix was unchanged (0.138 ms).
     KernelInfo kr;
     int rn = 0, rsm = 0;
     const bool runs_small = owned && !exact && plan->nsgroups > 0 &&

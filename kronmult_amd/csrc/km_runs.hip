@@ -556,6 +556,17 @@ __attribute__((amdgpu_waves_per_eu(runs_waves<T, D, NT, EXACT>()))) void kron_ru
       const int S = ipow_dev(n, D);
       // the regions are free: every wave is done with the previous run
       lds_barrier();
+      // outputs not owned (the per-size path's atomic calls, or a caller's
+      // pY that differs from the plan's): the run is still summed on chip
+      // when all its entries have the same output, and flushed with atomics
+      bool same = a.owned;
+      if (!a.owned)
+      {
+        int diff = 0;
+        for (int e2 = p + wave; e2 < re; e2 += G::NW)
+          diff |= (y_of(a, entry_at(a, e2)) != yb) ? 1 : 0;
+        same = !__syncthreads_or(diff);
+      }
       T acc[G::RS];
 #pragma unroll
       for (int q = 0; q < G::RS; q++)
@@ -581,10 +592,10 @@ __attribute__((amdgpu_waves_per_eu(runs_waves<T, D, NT, EXACT>()))) void kron_ru
         pass_sync<true>();
         runs_dispatch<T, D, NMAX, NT, EXACT, false>(n, wbuf, a, b, lane, acc);
         pass_sync<true>(); // the region is read before it is reloaded
-        if (!a.owned)
+        if (!same)
         {
-          // the caller's outputs differ from the plan's (plan_bind): each
-          // entry goes to its own output with atomics
+          // the run's entries have different outputs: each entry goes to
+          // its own output with atomics
           runs_flush<G::RS, 64>(y_of(a, b), acc, lane, S, 0);
 #pragma unroll
           for (int q = 0; q < G::RS; q++)
@@ -593,7 +604,7 @@ __attribute__((amdgpu_waves_per_eu(runs_waves<T, D, NT, EXACT>()))) void kron_ru
         b = bn;
         xp = xn;
       }
-      if (!a.owned)
+      if (!same)
         continue;
       const int used = min(G::NW, re - p);
       if (wave < used)

@@ -180,3 +180,43 @@ def test_grouped_owned_modes_with_regrouped_outputs(km, mode, d, sizes, groups):
                 assert np.array_equal(tY[n].cpu().numpy(), want[n]), (mode, n)
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("mode", ["deterministic", "persistent"])
+def test_grouped_owned_modes_on_another_output_buffer(km, mode):
+    """A group plan built for one output buffer and called with the same
+    grouping in another buffer: the check kernel flags the difference, every
+    output run still has one output, so the one-launch kernel sums each run on
+    chip and adds it with atomics (long n = 8 runs split into parts, n = 3 runs
+    over the wavefronts); the first buffer is left alone."""
+    import torch
+    dev = _dev()
+    d, sizes, batch, groups = 4, (3, 8), 200, 3
+    rng = np.random.default_rng(951)
+    ns = rng.choice(sizes, batch)
+    gg = rng.integers(0, groups, batch)
+    mats = [[rng.integers(-2, 3, (n, n)).astype(np.float64) for _ in range(d)] for n in ns]
+    X = [rng.integers(-2, 3, n ** d).astype(np.float64) for n in ns]
+    Y0 = {n: rng.integers(-4, 5, (groups, n ** d)).astype(np.float64) for n in sizes}
+    tYa = {n: torch.tensor(v, device=dev) for n, v in Y0.items()}
+    tYb = {n: torch.tensor(v, device=dev) for n, v in Y0.items()}
+    tX = [torch.tensor(x, device=dev) for x in X]
+    tA = [torch.tensor(np.ascontiguousarray(mats[b][k].T), device=dev) for b in range(batch) for k in range(d)]
+    table = torch.tensor([t.data_ptr() for t in tA], dtype=torch.int64, device=dev)
+    pX = torch.tensor([t.data_ptr() for t in tX], dtype=torch.int64, device=dev)
+
+    def ptrs(tY):
+        return torch.tensor([tY[n][gg[b]].data_ptr() for b, n in enumerate(ns)], dtype=torch.int64, device=dev)
+
+    want = {n: v.copy() for n, v in Y0.items()}
+    for b, n in enumerate(ns):
+        want[n][gg[b]] += O.brute(mats[b], X[b])
+    plan = km.GroupPlan(d, [int(v) for v in ns], ptrs(tYa))
+    try:
+        km.kronmult_grouped(plan, table, pX, ptrs(tYb), opt=km.options(**{mode: True}))
+        torch.cuda.synchronize()
+        for n in sizes:
+            assert np.array_equal(tYb[n].cpu().numpy(), want[n]), (mode, n)
+            assert np.array_equal(tYa[n].cpu().numpy(), Y0[n]), (mode, n)
+    finally:
+        plan.close()
