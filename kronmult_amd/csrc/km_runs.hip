@@ -405,13 +405,15 @@ __device__ __forceinline__ void runs_reg_unit(const EntryArgs<T> &a, const RunsA
   }
 }
 
-// With the register-blocked large entries (FP64), the kernel is compiled for
-// three workgroups per CU (80 VGPRs, 2 of them spilled), which its 40 KiB of
-// LDS allows; unconstrained it took 96 VGPRs (two workgroups per CU).
+// With the register-blocked large entries, the kernel is compiled for three
+// workgroups per CU in FP64 (80 VGPRs, a few spilled), which its 40 KiB of
+// LDS allows (unconstrained it took 96 VGPRs: two per CU), and for two in
+// float (128 VGPRs, 2 spilled; unconstrained 138: one per CU; three spilled
+// 34).
 template<typename T, int D, int NT, bool EXACT>
 constexpr int runs_waves()
 {
-  return (runs_reg_geom<D, NT, EXACT>() && sizeof(T) == 8) ? 6 : 1;
+  return runs_reg_geom<D, NT, EXACT>() ? (sizeof(T) == 8 ? 6 : 4) : 1;
 }
 
 template<typename T, int D, int NMAX, int NT, bool EXACT>
