@@ -406,9 +406,10 @@ __device__ __forceinline__ void runs_reg_unit(const EntryArgs<T> &a, const RunsA
 }
 
 // With the register-blocked large entries, the kernel is compiled for three
-// workgroups per CU in FP64 (80 VGPRs, a few spilled), which its 40 KiB of
-// LDS allows (unconstrained it took 96 VGPRs: two per CU), and for two in
-// float (128 VGPRs, 2 spilled; unconstrained 138: one per CU; three spilled
+// workgroups per CU in FP64 (at most 80 VGPRs; two values of the
+// double-buffered sum go to 32 bytes of scratch), which its 40 KiB of LDS
+// allows (unconstrained it took 96 VGPRs: two per CU), and for two in float
+// (at most 128 VGPRs; unconstrained 138: one per CU; three per CU spilled
 // 34).
 template<typename T, int D, int NT, bool EXACT>
 constexpr int runs_waves()
@@ -546,7 +547,7 @@ __attribute__((amdgpu_waves_per_eu(runs_waves<T, D, NT, EXACT>()))) void kron_ru
   {
     // the wavefront index is uniform over the wavefront: made explicit, so
     // everything derived from it (entries, pointers, factors) stays scalar
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane_in = tid & 63;
     T *wbuf = lds + wave * G::SW;
     for (int r = ub; r < u1; r++)
     {
@@ -578,6 +579,11 @@ __attribute__((amdgpu_waves_per_eu(runs_waves<T, D, NT, EXACT>()))) void kron_ru
       const T *xp = e < re ? x_of(a, b) : nullptr;
       for (; e < re; e += G::NW)
       {
+        // the lane through an empty asm statement: the per-lane offsets of
+        // every size's passes are computed per entry, not hoisted out of the
+        // loops (where they were spilled to scratch and reloaded per entry)
+        int lane = lane_in;
+        asm volatile("" : "+v"(lane));
         copy_to_lds<T, 64>(wbuf, xp, S, lane);
         // this wave's next entry of the run: pointers, its X and factors
         // into L2
@@ -614,7 +620,7 @@ __attribute__((amdgpu_waves_per_eu(runs_waves<T, D, NT, EXACT>()))) void kron_ru
 #pragma unroll
         for (int q = 0; q < G::RS; q++)
         {
-          const int i = lane + 64 * q;
+          const int i = lane_in + 64 * q;
           if (i < S)
             wbuf[i] = acc[q];
         }
