@@ -5,6 +5,8 @@ without a plan, FP64 or float, and for xbatched a padded lda, a small A pool
 and a random subbatchCount; every call against the CPU oracle."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -14,6 +16,10 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(d, n) for d in range(1, 7) for n in range(1, 9) if n ** d <= 262144]
+
+# KM_RANDOM_EXTRA_SEEDS=K adds K more seeds to the call and grouped sweeps
+# (a longer run of the same generators: 200 calls or 15 grouped calls per seed)
+EXTRA = int(os.environ.get("KM_RANDOM_EXTRA_SEEDS", "0"))
 
 
 def _case(km, rng):
@@ -88,7 +94,7 @@ def km():
     return kronmult_amd
 
 
-@pytest.mark.parametrize("seed", [11, 23, 37])
+@pytest.mark.parametrize("seed", [11, 23, 37] + list(range(1000, 1000 + EXTRA)))
 def test_random_calls_vs_oracle(km, seed):
     rng = np.random.default_rng(seed)
     ran = 0
@@ -132,7 +138,7 @@ def test_random_vbatched_vs_oracle(km, seed):
         assert rel_err(got, want) <= tol, (m, n, batch, f32, exact, variant, rel_err(got, want))
 
 
-@pytest.mark.parametrize("seed", [3, 29])
+@pytest.mark.parametrize("seed", [3, 29] + list(range(2000, 2000 + EXTRA)))
 def test_random_grouped_vs_oracle(km, seed):
     """Grouped calls with random d, random per-entry sizes (within each d's
     supported range) and random output groups, in every launch mode."""
