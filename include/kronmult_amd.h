@@ -272,7 +272,7 @@ km_status km_plan_info(km_plan plan, int *num_outputs, int *num_groups);
  * on one plan take turns through the fork (a per-plan lock).  KM_FLAG_PERSISTENT runs all bins in
  * one launch with owned outputs (each workgroup takes whole output runs;
  * bitwise reproducible; the fastest owned mode on mixed sizes, within about
- * 1.1x of the default on a d = 4 mix of n = 2..8).  An output run of large entries too long for one workgroup is
+ * 1.05x of the default on a d = 4 mix of n = 2..8).  An output run of large entries too long for one workgroup is
  * cut into parts whose sums the plan keeps in its own device buffer and a
  * second short launch adds to the output in part order, so such a call is
  * two launches, and persistent calls of one plan must not overlap (on
